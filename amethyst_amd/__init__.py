@@ -1,0 +1,295 @@
+"""MI355X (gfx950) packed-column operator engine -- Python host side.
+
+A thin mirror of the C ABI in ``include/amethyst.h`` over ctypes. PyTorch is
+used only for device memory and streams; every operator runs in the HIP
+library ``amethyst_amd/libamethyst.so``. There is no CPU fallback: if the
+library is missing, ``lib()`` raises.
+
+The reference's operator surface is templated CUDA
+(``kernel_<op>_<format><block_size, bits, [outbits], op>``, src/kernels/*,
+host wrappers in src/lib.cuh:4-46). Here format, width and operator are
+runtime arguments of one entry point per operator.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamethyst.so")
+_LIB = None
+
+# formats (include/amethyst.h)
+BASELINE, SLAB, BITSTREAM, TIGHT, U32 = 0, 1, 2, 3, 4
+FORMAT_NAMES = {"baseline": BASELINE, "slab": SLAB, "bitstream": BITSTREAM, "tight": TIGHT, "u32": U32}
+# predicates
+LT, LE, GT, GE, EQ, NE = range(6)
+# binary ops
+(ADD, MUL, MOD, DIV, MATH, BOOL_SIMPLE, BOOL_COMPLEX, AND, OR, XOR, SUB) = range(11)
+# group-by modes
+GB_EXACT, GB_REF = 0, 1
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported configuration", 3: "HIP runtime error",
+          4: "workspace missing or too small", 5: "output capacity too small"}
+
+
+class AmethystError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {STATUS.get(code, code)} (status {code})")
+        self.code = code
+
+
+class am_column(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("n", ctypes.c_uint64), ("bits", ctypes.c_uint32),
+                ("format", ctypes.c_uint32)]
+
+
+class am_predicate(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("c", ctypes.c_uint64)]
+
+
+def build(verbose=False):
+    """Compile the HIP library in-tree for gfx950 (hipcc via make)."""
+    import subprocess
+    cmd = ["make", "-C", os.path.join(_HERE, "csrc"), "-j8"]
+    if not verbose:
+        cmd.insert(1, "-s")
+    subprocess.check_call(cmd)
+
+
+def lib():
+    """Load libamethyst.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run amethyst_amd.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+        colp = ctypes.POINTER(am_column)
+        predp = ctypes.POINTER(am_predicate)
+        sigs = {
+            "am_abi_version": ([], i32),
+            "am_status_string": ([i32], ctypes.c_char_p),
+            "am_packed_words": ([u32, u32, u64], u64),
+            "am_outbits": ([u32, u32], u32),
+            "am_pred_width": ([am_predicate, u32], u32),
+            "am_gen_bitmask": ([u32], u64),
+            "am_workspace_bytes": ([u64], u64),
+            "am_workspace_init": ([vp, u64, vp], i32),
+            "am_prng_fill_u32": ([u64, u64, vp, u64, vp], i32),
+            "am_gen_uniform": ([u64, colp, vp], i32),
+            "am_gen_from_cdf": ([u64, vp, u32, colp, vp], i32),
+            "am_gen_mask_bernoulli": ([u64, ctypes.c_double, vp, u64, vp], i32),
+            "am_gen_mask_runs": ([u64, ctypes.c_double, ctypes.c_double, vp, u64, vp], i32),
+            "am_convert": ([colp, colp, vp], i32),
+            "am_filter": ([colp, am_predicate, vp, vp], i32),
+            "am_compressstore": ([colp, vp, predp, colp, vp, vp, vp, u64, vp], i32),
+            "am_binary": ([u32, colp, colp, colp, vp], i32),
+            "am_sum": ([colp, vp, vp, u64, vp], i32),
+        }
+        for name, (args, res) in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    """Names of the C entry points this module binds."""
+    lib()
+    return [n for n in dir(_LIB) if n.startswith("am_")]
+
+
+def _check(code, what):
+    if code != 0:
+        raise AmethystError(code, what)
+
+
+# ----------------------------------------------------------------- pure helpers
+
+def packed_words(fmt, bits, n):
+    return int(lib().am_packed_words(fmt, bits, n))
+
+
+def outbits(op, bits):
+    return int(lib().am_outbits(op, bits))
+
+
+def pred_width(op, c, bits):
+    return int(lib().am_pred_width(am_predicate(op, 0, c), bits))
+
+
+def workspace_bytes(max_elements):
+    return int(lib().am_workspace_bytes(max_elements))
+
+
+# ------------------------------------------------------------- device columns
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def device_words(words, device="cuda"):
+    """A zero-filled device buffer of `words` uint64 words (int64 storage)."""
+    torch = _torch()
+    return torch.zeros(max(1, int(words)), dtype=torch.int64, device=device)
+
+
+class Column:
+    """`n` values of `bits` bits in `fmt`, held in a device buffer of uint64 words."""
+
+    def __init__(self, n, bits, fmt, buf=None, device="cuda"):
+        self.n, self.bits, self.fmt = int(n), int(bits), int(fmt)
+        w = packed_words(self.fmt, self.bits, self.n)
+        if w == 0 and self.n:
+            raise ValueError(f"invalid format/width: fmt={fmt} bits={bits}")
+        self.buf = buf if buf is not None else device_words(w, device)
+        if self.buf.numel() < max(1, w):
+            raise ValueError("buffer too small for the column")
+
+    @property
+    def words(self):
+        return packed_words(self.fmt, self.bits, self.n)
+
+    def c(self):
+        return am_column(self.buf.data_ptr(), self.n, self.bits, self.fmt)
+
+    def host_words(self):
+        """Packed words copied to the host as a numpy uint64 array."""
+        return self.buf[:self.words].cpu().numpy().view(np.uint64).copy()
+
+    @classmethod
+    def from_host_words(cls, words, n, bits, fmt, device="cuda"):
+        torch = _torch()
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        need = packed_words(fmt, bits, n)
+        buf = torch.zeros(max(1, need), dtype=torch.int64, device=device)
+        if need:
+            buf[:need].copy_(torch.from_numpy(w[:need].view(np.int64)))
+        return cls(n, bits, fmt, buf=buf)
+
+
+class Workspace:
+    """Zero-initialised workspace for the single-pass operators."""
+
+    def __init__(self, max_elements, device="cuda"):
+        self.bytes = workspace_bytes(max_elements)
+        self.buf = device_words((self.bytes + 7) // 8, device)  # torch.zeros: already zero
+
+    def ptr(self):
+        return ctypes.c_void_p(self.buf.data_ptr())
+
+
+# ----------------------------------------------------------------- operators
+
+def prng_fill(seed, n, offset=0, stream=None):
+    torch = _torch()
+    out = torch.empty(max(1, n), dtype=torch.int32, device="cuda")
+    _check(lib().am_prng_fill_u32(seed, offset, ctypes.c_void_p(out.data_ptr()), n, _stream(stream)),
+           "am_prng_fill_u32")
+    return out[:n]
+
+
+def gen_uniform(seed, n, bits, fmt, stream=None):
+    col = Column(n, bits, fmt)
+    c = col.c()
+    _check(lib().am_gen_uniform(seed, ctypes.byref(c), _stream(stream)), "am_gen_uniform")
+    return col
+
+
+def gen_from_cdf(seed, thresholds, n, bits, fmt, stream=None):
+    """Values drawn by inverse CDF: value = first b with draw < thresholds[b]."""
+    torch = _torch()
+    thr = torch.as_tensor(np.ascontiguousarray(thresholds, dtype=np.uint32).view(np.int32)).cuda()
+    col = Column(n, bits, fmt)
+    c = col.c()
+    _check(lib().am_gen_from_cdf(seed, ctypes.c_void_p(thr.data_ptr()), len(thresholds), ctypes.byref(c),
+                                 _stream(stream)), "am_gen_from_cdf")
+    torch.cuda.current_stream().synchronize()
+    return col
+
+
+def gen_mask_bernoulli(seed, n, selectivity, stream=None):
+    m = device_words((n + 63) // 64)
+    _check(lib().am_gen_mask_bernoulli(seed, selectivity, ctypes.c_void_p(m.data_ptr()), n, _stream(stream)),
+           "am_gen_mask_bernoulli")
+    return m
+
+
+def gen_mask_runs(seed, n, selectivity, mean_run=256.0, stream=None):
+    m = device_words((n + 63) // 64)
+    _check(lib().am_gen_mask_runs(seed, selectivity, mean_run, ctypes.c_void_p(m.data_ptr()), n, _stream(stream)),
+           "am_gen_mask_runs")
+    return m
+
+
+def convert(col, bits, fmt, stream=None):
+    out = Column(col.n, bits, fmt)
+    a, o = col.c(), out.c()
+    _check(lib().am_convert(ctypes.byref(a), ctypes.byref(o), _stream(stream)), "am_convert")
+    return out
+
+
+def filter(col, op, c, mask_out=None, stream=None):
+    """Predicate filter -> LSB-first bitmask (device int64 buffer of ceil(n/64) words)."""
+    m = mask_out if mask_out is not None else device_words((col.n + 63) // 64)
+    a = col.c()
+    _check(lib().am_filter(ctypes.byref(a), am_predicate(op, 0, c), ctypes.c_void_p(m.data_ptr()), _stream(stream)),
+           "am_filter")
+    return m
+
+
+def compressstore(col, out_bits, out_fmt, mask=None, pred=None, want_mask=False, capacity=None,
+                  ws=None, out=None, stats=None, mask_out=None, stream=None):
+    """Compaction; returns (out Column (n = capacity), stats device tensor, mask or None).
+
+    stats holds {selected count, values that did not fit out_bits}.
+    """
+    if (mask is None) == (pred is None):
+        raise ValueError("give exactly one of mask / pred")
+    cap = col.n if capacity is None else int(capacity)
+    out = out if out is not None else Column(cap, out_bits, out_fmt)
+    ws = ws if ws is not None else Workspace(col.n)
+    stats = stats if stats is not None else device_words(2)
+    if want_mask and mask_out is None:
+        mask_out = device_words((col.n + 63) // 64)
+    a, o = col.c(), out.c()
+    pp = ctypes.byref(am_predicate(pred[0], 0, pred[1])) if pred is not None else None
+    mptr = ctypes.c_void_p(mask.data_ptr()) if mask is not None else None
+    moptr = ctypes.c_void_p(mask_out.data_ptr()) if mask_out is not None else None
+    _check(lib().am_compressstore(ctypes.byref(a), mptr, pp, ctypes.byref(o), moptr,
+                                  ctypes.c_void_p(stats.data_ptr()), ws.ptr(), ws.bytes, _stream(stream)),
+           "am_compressstore")
+    return out, stats, mask_out
+
+
+def binary(op, a, b, out_bits=None, out_fmt=None, out=None, stream=None):
+    ob = outbits(op, a.bits) if out_bits is None else out_bits
+    of = a.fmt if out_fmt is None else out_fmt
+    out = out if out is not None else Column(a.n, ob, of)
+    ca, cb, co = a.c(), b.c(), out.c()
+    _check(lib().am_binary(op, ctypes.byref(ca), ctypes.byref(cb), ctypes.byref(co), _stream(stream)), "am_binary")
+    return out
+
+
+def sum(col, ws=None, out=None, stream=None):
+    """Wrapping uint64 sum; returns a 1-word device tensor."""
+    ws = ws if ws is not None else Workspace(0)
+    out = out if out is not None else device_words(1)
+    a = col.c()
+    _check(lib().am_sum(ctypes.byref(a), ctypes.c_void_p(out.data_ptr()), ws.ptr(), ws.bytes, _stream(stream)),
+           "am_sum")
+    return out
+
+
+def u64(t):
+    """Device int64 tensor -> host numpy uint64 array."""
+    return t.cpu().numpy().view(np.uint64)

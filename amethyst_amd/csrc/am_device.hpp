@@ -1,0 +1,246 @@
+// am_device.hpp -- device-side building blocks shared by the gfx950 kernels:
+// column-format decoders, wave64 primitives, and the single-pass look-back
+// state used by compaction.
+//
+// Formats follow the reference layouts (src/common_code.cuh:11-19,43-55,
+// 100-116). Internally the five public formats reduce to three "kinds":
+// STRIDED (one element per 1/2/4/8-byte container: BASELINE, TIGHT, U32),
+// SLAB (floor(64/bits) elements per uint64, dead bits on top) and
+// BITSTREAM (dense).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/amethyst.h"
+
+namespace am {
+
+enum Kind : int { K_STRIDED = 0, K_SLAB = 1, K_BITSTREAM = 2 };
+
+__host__ __device__ inline uint64_t lowmask(uint32_t bits)
+{
+    return bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
+}
+
+// Column parameters resolved on the host (see capi.cpp: make_col).
+struct Col {
+    uint64_t* p;        // packed data (uint64 words)
+    uint64_t n;         // elements
+    uint64_t mask;      // lowmask(bits)
+    uint32_t bits;
+    uint32_t kind;      // Kind
+    uint32_t cbytes;    // STRIDED container bytes (1, 2, 4, 8)
+    uint32_t eps;       // SLAB elements per word
+    uint32_t eps_magic; // SLAB: floor(2^32/eps)+1, exact quotient for idx < 2^24
+    uint32_t pad;
+};
+
+// 32-bit quotient idx/eps for idx < 2^24 and eps in 2..64 (multiply-high by
+// floor(2^32/eps)+1: the error term stays below 2^-8 < 1/eps).
+__device__ inline uint32_t div_eps(uint32_t idx, const Col& c)
+{
+    return c.eps == 1 ? idx : __umulhi(idx, c.eps_magic);
+}
+
+// A decoder for elements [t0, t0 + 2^24) of a column: per-tile constants are
+// formed once (scalar 64-bit math), per-element work is 32-bit.
+template <int KIND>
+struct TileReader {
+    const uint64_t* base; // first word touched by the tile
+    uint32_t r0;          // SLAB: t0 % eps ; BITSTREAM: (t0*bits) % 64
+    uint64_t mask;
+    uint32_t bits, cbytes, eps, eps_magic;
+    Col c;
+
+    __device__ inline TileReader(const Col& col, uint64_t t0) : c(col)
+    {
+        mask = col.mask;
+        bits = col.bits;
+        cbytes = col.cbytes;
+        eps = col.eps;
+        eps_magic = col.eps_magic;
+        if (KIND == K_STRIDED) {
+            base = (const uint64_t*)((const uint8_t*)col.p + t0 * col.cbytes);
+            r0 = 0;
+        }
+        else if (KIND == K_SLAB) {
+            uint64_t q = t0 / col.eps;
+            base = col.p + q;
+            r0 = (uint32_t)(t0 - q * col.eps);
+        }
+        else {
+            uint64_t bit = t0 * (uint64_t)col.bits;
+            base = col.p + (bit >> 6);
+            r0 = (uint32_t)(bit & 63);
+        }
+    }
+
+    // element t0 + i (caller guarantees it is < n)
+    __device__ inline uint64_t get(uint32_t i) const
+    {
+        if (KIND == K_STRIDED) {
+            switch (cbytes) {
+                case 1: return ((const uint8_t*)base)[i] & mask;
+                case 2: return ((const uint16_t*)base)[i] & mask;
+                case 4: return ((const uint32_t*)base)[i] & mask;
+                default: return base[i] & mask;
+            }
+        }
+        else if (KIND == K_SLAB) {
+            uint32_t idx = r0 + i;
+            uint32_t q = eps == 1 ? idx : __umulhi(idx, eps_magic);
+            uint32_t s = idx - q * eps;
+            uint64_t w = base[q];
+            return (w >> (s * bits)) & mask;
+        }
+        else {
+            uint32_t lb = r0 + i * bits;
+            const uint64_t* p = base + (lb >> 6);
+            uint32_t o = lb & 63;
+            uint64_t v = p[0] >> o;
+            if (o + bits > 64) v |= p[1] << (64 - o);
+            return v & mask;
+        }
+    }
+};
+
+// ------------------------------------------------------------ wave64 helpers
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// number of set bits of m below this lane (v_mbcnt_lo/hi)
+__device__ inline uint32_t mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int m)
+{
+    uint32_t lo = __shfl_xor((unsigned)(uint32_t)v, m);
+    uint32_t hi = __shfl_xor((unsigned)(uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint64_t wave_sum_u64(uint64_t v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_u64(v, m);
+    return v;
+}
+
+__device__ inline uint64_t wave_or_u64(uint64_t v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v |= shfl_xor_u64(v, m);
+    return v;
+}
+
+__device__ inline uint64_t wave_max_u64(uint64_t v)
+{
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint64_t o = shfl_xor_u64(v, m);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ inline bool pred_eval(uint32_t op, uint64_t c, uint64_t v)
+{
+    switch (op) {
+        case AM_PRED_LT: return v < c;
+        case AM_PRED_LE: return v <= c;
+        case AM_PRED_GT: return v > c;
+        case AM_PRED_GE: return v >= c;
+        case AM_PRED_EQ: return v == c;
+        default: return v != c;
+    }
+}
+
+// ------------------------------------------------------------ agent-scope atomics
+
+__device__ inline uint64_t ld_acq(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_rel(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t ld32(const uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st32(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t add32(uint32_t* p, uint32_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t add64(uint64_t* p, uint64_t v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every memory operation this wave issued has completed
+__device__ inline void drain() { __builtin_amdgcn_s_waitcnt(0); }
+
+// ------------------------------------------------------------ workspace
+
+// Workspace header; followed by the group-by table and the look-back
+// descriptors. Every kernel that uses it returns the counters to zero.
+struct WsHeader {
+    uint32_t ticket;   // dynamic tile id counter
+    uint32_t done;     // finished-block counter
+    uint32_t epoch;    // launch counter of the look-back kernels (parity)
+    uint32_t lastT[2]; // tiles used by the last look-back launch of each parity
+    uint32_t pad0[3];
+    uint64_t overflow; // compaction: values wider than the output width
+    uint64_t accum;    // sum
+    uint64_t rows;     // group-by rows aggregated
+    uint64_t pad1[25];
+};
+static_assert(sizeof(WsHeader) == 256, "header size");
+
+constexpr uint64_t WS_GB_ENTRIES = 4096; // group-by table, keys <= 12 bits
+constexpr uint64_t WS_GB_BYTES = WS_GB_ENTRIES * 16;
+constexpr uint64_t WS_FIXED = 256 + WS_GB_BYTES;
+constexpr uint64_t WS_DESC_BYTES_PER_TILE = 6 * 8; // 3 arrays x 2 parities
+
+struct Lookback {
+    uint64_t* status; // [2][cap]  bits 63..62 state (1 = aggregate, 2 = inclusive), 61..0 count
+    uint64_t* tailA;  // [2][cap]  bit 63 valid, bits 62..0 window of the tile's own values
+    uint64_t* tailP;  // [2][cap]  bit 63 valid, bits 62..0 window of the inclusive prefix
+    uint64_t cap;
+};
+
+__host__ __device__ inline Lookback lookback_of(void* ws, uint64_t ws_bytes)
+{
+    Lookback L;
+    uint64_t cap = (ws_bytes - WS_FIXED) / WS_DESC_BYTES_PER_TILE;
+    uint64_t* d = (uint64_t*)((uint8_t*)ws + WS_FIXED);
+    L.status = d;
+    L.tailA = d + 2 * cap;
+    L.tailP = d + 4 * cap;
+    L.cap = cap;
+    return L;
+}
+
+constexpr uint64_t ST_A = 1ull << 62;
+constexpr uint64_t ST_P = 2ull << 62;
+constexpr uint64_t ST_COUNT = (1ull << 62) - 1;
+constexpr uint64_t TAIL_VALID = 1ull << 63;
+constexpr uint64_t TAIL_BITS = (1ull << 63) - 1;
+
+// Scan element of the compaction: (count, last-63-bits window of the dense
+// stream of selected values at the output width). Windows are right-aligned:
+// bit 62 is the last stream bit. Appending y (m = y.count*bits stream bits)
+// to x gives (x.win >> m) | y.win when m < 63, else y.win.
+__device__ inline uint64_t win_combine(uint64_t xw, uint64_t yw, uint64_t ycount, uint32_t bits)
+{
+    uint64_t m = ycount * bits;
+    if (m >= 63) return yw & TAIL_BITS;
+    return ((xw & TAIL_BITS) >> m) | (yw & TAIL_BITS);
+}
+
+} // namespace am

@@ -1,0 +1,57 @@
+// am_host.hpp -- host-side helpers shared by the C-ABI entry points:
+// column validation and resolution into the device-side Col descriptor.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/amethyst.h"
+#include "am_device.hpp"
+
+namespace am {
+
+inline uint32_t tight_bytes(uint32_t bits)
+{
+    // tight_type<bits>::type, src/common_code.cuh:11-19
+    uint32_t bc = (bits + 7) / 8;
+    return bc > 4 ? 8 : (bc > 2 ? 4 : (bc > 1 ? 2 : 1));
+}
+
+inline bool valid_format(uint32_t fmt, uint32_t bits)
+{
+    if (bits < 1 || bits > 64) return false;
+    if (fmt == AM_FMT_U32) return bits <= 32;
+    return fmt <= AM_FMT_U32;
+}
+
+// Resolve a public column into the device descriptor (AM_OK or an error).
+inline int make_col(const am_column* c, Col* out)
+{
+    if (!c) return AM_E_INVALID;
+    if (!valid_format(c->format, c->bits)) return AM_E_INVALID;
+    if (c->n && !c->data) return AM_E_INVALID;
+    if (c->n >= (1ull << 40)) return AM_E_UNSUPPORTED_CONFIG;
+    Col d{};
+    d.p = (uint64_t*)c->data;
+    d.n = c->n;
+    d.bits = c->bits;
+    d.mask = lowmask(c->bits);
+    d.eps = 64 / c->bits;
+    d.eps_magic = d.eps > 1 ? (uint32_t)((1ull << 32) / d.eps + 1) : 0;
+    switch (c->format) {
+        case AM_FMT_BASELINE: d.kind = K_STRIDED; d.cbytes = 8; break;
+        case AM_FMT_TIGHT: d.kind = K_STRIDED; d.cbytes = tight_bytes(c->bits); break;
+        case AM_FMT_U32: d.kind = K_STRIDED; d.cbytes = 4; break;
+        case AM_FMT_SLAB: d.kind = K_SLAB; d.cbytes = 8; break;
+        default: d.kind = K_BITSTREAM; d.cbytes = 8; break;
+    }
+    *out = d;
+    return AM_OK;
+}
+
+inline int check_launch()
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? AM_OK : AM_E_HIP;
+}
+
+} // namespace am

@@ -1,0 +1,95 @@
+// capi.hip -- helper entry points of the C ABI (sizes, width rules,
+// workspace set-up). Operators live with their kernels.
+#include "am_device.hpp"
+#include "am_host.hpp"
+
+namespace am {
+
+__global__ __launch_bounds__(256) void k_zero_words(uint64_t* p, uint64_t words)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+} // namespace am
+
+using namespace am;
+
+extern "C" int am_abi_version(void) { return AM_ABI_VERSION; }
+
+extern "C" const char* am_status_string(int status)
+{
+    switch (status) {
+        case AM_OK: return "ok";
+        case AM_E_INVALID: return "invalid argument";
+        case AM_E_UNSUPPORTED_CONFIG: return "unsupported configuration";
+        case AM_E_HIP: return "HIP runtime error";
+        case AM_E_WORKSPACE: return "workspace missing or too small";
+        case AM_E_CAPACITY: return "output capacity too small";
+    }
+    return "unknown status";
+}
+
+extern "C" uint64_t am_packed_words(uint32_t format, uint32_t bits, uint64_t n)
+{
+    if (!valid_format(format, bits)) return 0;
+    switch (format) {
+        case AM_FMT_BASELINE: return n;
+        case AM_FMT_SLAB: {
+            uint64_t eps = 64 / bits;
+            return (n + eps - 1) / eps;
+        }
+        case AM_FMT_BITSTREAM: return (n * bits + 63) / 64;
+        case AM_FMT_TIGHT: return (n * tight_bytes(bits) + 7) / 8;
+        default: return (n + 1) / 2;
+    }
+}
+
+extern "C" uint32_t am_outbits(uint32_t op, uint32_t bits)
+{
+    // op_add_outbits / op_mul_outbits / op_mod_outbits, src/config.cuh:44-55
+    if (op == AM_OP_ADD) return bits >= 64 ? 64 : bits + 1;
+    if (op == AM_OP_MUL) return bits * 2 >= 64 ? 64 : bits * 2;
+    return bits;
+}
+
+static uint32_t width_of(uint64_t v)
+{
+    uint32_t w = 1;
+    while (w < 64 && (v >> w)) w++;
+    return w;
+}
+
+extern "C" uint32_t am_pred_width(am_predicate pred, uint32_t bits)
+{
+    uint64_t vmax = lowmask(bits);
+    switch (pred.op) {
+        case AM_PRED_LT:
+            if (pred.c == 0) return 1;
+            vmax = pred.c - 1 < vmax ? pred.c - 1 : vmax;
+            break;
+        case AM_PRED_LE: vmax = pred.c < vmax ? pred.c : vmax; break;
+        case AM_PRED_EQ: vmax = pred.c & vmax; break;
+        default: break;
+    }
+    return width_of(vmax);
+}
+
+extern "C" uint64_t am_gen_bitmask(uint32_t bits) { return bits == 0 ? 0 : lowmask(bits); }
+
+extern "C" uint64_t am_workspace_bytes(uint64_t max_elements)
+{
+    // one descriptor set per 4096-element tile (the compaction tile), plus one
+    uint64_t tiles = (max_elements + 4095) / 4096 + 1;
+    return WS_FIXED + tiles * WS_DESC_BYTES_PER_TILE;
+}
+
+extern "C" int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream)
+{
+    if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
+    uint64_t words = ws_bytes / 8;
+    uint64_t blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_words, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, (uint64_t*)ws, words);
+    return check_launch();
+}

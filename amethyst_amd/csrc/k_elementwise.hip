@@ -1,0 +1,159 @@
+// k_elementwise.hip -- format conversion and packed binary operators.
+//
+// Reference: kernel_binary_op_baseline / _nogaps / _slabs
+// (src/kernels/binary_op_baseline.cuh:4-13, binary_op_nogaps.cuh:5-89,
+// binary_op_slabs.cuh:7-67). The reference stages input slabs in shared
+// memory and has each thread assemble one output word; its operator is a
+// compile-time function pointer masked to the out-width (src/config.cuh:5-42).
+//
+// Here one lane also owns one output word (so every store is a coalesced,
+// whole-word write and dead/padding bits are written as zero), but input
+// elements are decoded straight from global memory through a per-block
+// TileReader: neighbouring lanes touch neighbouring input words, so each load
+// instruction covers one or two 128-byte lines and the L1 absorbs the reuse
+// between lanes. Any input format x any output format; the operator is a
+// template argument so the switch folds away.
+#include "am_device.hpp"
+#include "am_host.hpp"
+
+namespace am {
+
+__device__ inline uint64_t udiv(uint64_t l, uint64_t r) { return r ? l / r : ~0ull; }
+__device__ inline uint64_t umod(uint64_t l, uint64_t r) { return r ? l % r : l; }
+
+template <int OP>
+__device__ inline uint64_t binop(uint64_t l, uint64_t r)
+{
+    if (OP == AM_OP_ADD) return l + r;
+    if (OP == AM_OP_MUL) return l * r;
+    if (OP == AM_OP_MOD) return umod(l, r);
+    if (OP == AM_OP_DIV) return udiv(l, r);
+    if (OP == AM_OP_MATH) return udiv(l * l + r * r + 4 * l, l * l + r * r + 5 * l);
+    if (OP == AM_OP_BOOL_SIMPLE) return (l || r) ? 1 : 0;
+    if (OP == AM_OP_BOOL_COMPLEX) return ((l && !r) || (!l && r)) ? 1 : 0;
+    if (OP == AM_OP_AND) return l & r;
+    if (OP == AM_OP_OR) return l | r;
+    if (OP == AM_OP_XOR) return l ^ r;
+    return l - r; // AM_OP_SUB
+}
+
+constexpr int OP_COPY = -1; // conversion: out = in & mask(out.bits)
+constexpr uint32_t EW_UNITS = 1024; // output units (elements or words) per block
+
+template <int AK, int OK, int OP>
+__global__ __launch_bounds__(256) void k_elementwise(Col a, Col b, Col out)
+{
+    const uint64_t n = out.n;
+    const uint32_t bo = out.bits;
+    const uint64_t om = out.mask;
+    const uint64_t u0 = (uint64_t)blockIdx.x * EW_UNITS;
+    uint64_t t0; // first element touched by this block
+    if (OK == K_STRIDED) t0 = u0;
+    else if (OK == K_SLAB) t0 = u0 * out.eps;
+    else t0 = (u0 * 64) / bo;
+    if (t0 >= n) return;
+    TileReader<AK> ra(a, t0);
+    TileReader<AK> rb(OP == OP_COPY ? a : b, t0);
+    auto value = [&](uint32_t i) -> uint64_t {
+        if (OP == OP_COPY) return ra.get(i) & om;
+        return binop<OP>(ra.get(i), rb.get(i)) & om;
+    };
+    for (uint32_t r = 0; r < EW_UNITS / 256; r++) {
+        const uint64_t u = u0 + r * 256 + threadIdx.x;
+        if (OK == K_STRIDED) {
+            if (u >= n) break;
+            uint64_t v = value((uint32_t)(u - t0));
+            switch (out.cbytes) {
+                case 1: ((uint8_t*)out.p)[u] = (uint8_t)v; break;
+                case 2: ((uint16_t*)out.p)[u] = (uint16_t)v; break;
+                case 4: ((uint32_t*)out.p)[u] = (uint32_t)v; break;
+                default: out.p[u] = v;
+            }
+        }
+        else if (OK == K_SLAB) {
+            const uint64_t e0 = u * out.eps;
+            if (e0 >= n) break;
+            const uint32_t cnt = (uint32_t)(n - e0 < out.eps ? n - e0 : out.eps);
+            uint64_t word = 0;
+            for (uint32_t s = 0; s < cnt; s++) word |= value((uint32_t)(e0 - t0) + s) << (s * bo);
+            out.p[u] = word;
+        }
+        else {
+            const uint64_t b0 = u * 64;
+            if (b0 >= n * bo) break;
+            uint64_t e0 = b0 / bo, e1 = (b0 + 63) / bo + 1;
+            if (e1 > n) e1 = n;
+            uint64_t word = 0;
+            for (uint64_t e = e0; e < e1; e++) {
+                int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
+                uint64_t v = value((uint32_t)(e - t0));
+                word |= pos >= 0 ? (v << pos) : (v >> (-pos));
+            }
+            out.p[u] = word;
+        }
+    }
+}
+
+template <int OP>
+static void launch_ew(const Col& a, const Col& b, const Col& o, hipStream_t s)
+{
+    uint64_t units = o.kind == K_STRIDED ? o.n : (o.kind == K_SLAB ? (o.n + o.eps - 1) / o.eps : (o.n * o.bits + 63) / 64);
+    dim3 grid((uint32_t)((units + EW_UNITS - 1) / EW_UNITS));
+#define EW_CASE(AK, OK) hipLaunchKernelGGL((k_elementwise<AK, OK, OP>), grid, dim3(256), 0, s, a, b, o)
+#define EW_OUT(AK)                                                  \
+    switch (o.kind) {                                               \
+        case K_STRIDED: EW_CASE(AK, K_STRIDED); break;              \
+        case K_SLAB: EW_CASE(AK, K_SLAB); break;                    \
+        default: EW_CASE(AK, K_BITSTREAM); break;                   \
+    }
+    switch (a.kind) {
+        case K_STRIDED: EW_OUT(K_STRIDED); break;
+        case K_SLAB: EW_OUT(K_SLAB); break;
+        default: EW_OUT(K_BITSTREAM); break;
+    }
+#undef EW_OUT
+#undef EW_CASE
+}
+
+} // namespace am
+
+using namespace am;
+
+extern "C" int am_convert(const am_column* in, am_column* out, void* stream)
+{
+    Col a, o;
+    int st = make_col(in, &a);
+    if (st) return st;
+    if ((st = make_col(out, &o))) return st;
+    if (a.n != o.n) return AM_E_INVALID;
+    if (o.n == 0) return AM_OK;
+    launch_ew<OP_COPY>(a, a, o, (hipStream_t)stream);
+    return check_launch();
+}
+
+extern "C" int am_binary(uint32_t op, const am_column* a, const am_column* b, am_column* out, void* stream)
+{
+    Col ca, cb, o;
+    int st = make_col(a, &ca);
+    if (st) return st;
+    if ((st = make_col(b, &cb))) return st;
+    if ((st = make_col(out, &o))) return st;
+    if (ca.n != cb.n || ca.n != o.n || a->bits != b->bits || a->format != b->format) return AM_E_INVALID;
+    if (o.n == 0) return AM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    switch (op) {
+        case AM_OP_ADD: launch_ew<AM_OP_ADD>(ca, cb, o, s); break;
+        case AM_OP_MUL: launch_ew<AM_OP_MUL>(ca, cb, o, s); break;
+        case AM_OP_MOD: launch_ew<AM_OP_MOD>(ca, cb, o, s); break;
+        case AM_OP_DIV: launch_ew<AM_OP_DIV>(ca, cb, o, s); break;
+        case AM_OP_MATH: launch_ew<AM_OP_MATH>(ca, cb, o, s); break;
+        case AM_OP_BOOL_SIMPLE: launch_ew<AM_OP_BOOL_SIMPLE>(ca, cb, o, s); break;
+        case AM_OP_BOOL_COMPLEX: launch_ew<AM_OP_BOOL_COMPLEX>(ca, cb, o, s); break;
+        case AM_OP_AND: launch_ew<AM_OP_AND>(ca, cb, o, s); break;
+        case AM_OP_OR: launch_ew<AM_OP_OR>(ca, cb, o, s); break;
+        case AM_OP_XOR: launch_ew<AM_OP_XOR>(ca, cb, o, s); break;
+        case AM_OP_SUB: launch_ew<AM_OP_SUB>(ca, cb, o, s); break;
+        default: return AM_E_INVALID;
+    }
+    return check_launch();
+}

@@ -1,0 +1,221 @@
+/*
+ * amethyst.h -- C ABI of the MI355X (gfx950) packed-column operator engine.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream
+ * passed as `void*` (a hipStream_t; NULL = the default stream). Nothing here
+ * allocates device memory except the helpers named *_create; every operator
+ * is stream-ordered and asynchronous, reentrant, and keeps no global state.
+ * Errors are returned as AM_* status codes; nothing aborts the process.
+ *
+ * The reference (yogi-tud/Amethyst) is a CUDA benchmark whose operators are
+ * templated kernels `kernel_<op>_<format><block_size, bits, [outbits], op>`
+ * launched by per-bench host functions; its intended host wrappers are in
+ * src/lib.cuh:4-46 (never compiled there). Each declaration below names the
+ * reference kernel/wrapper it replaces.
+ *
+ * Column formats (reference layouts, src/common_code.cuh):
+ *   AM_FMT_BASELINE  one uint64 per element, value = word & mask(bits)
+ *                    (get_element_baseline, src/common_code.cuh:51-55)
+ *   AM_FMT_SLAB      "element stuffing": floor(64/bits) elements per uint64,
+ *                    element i in word i/eps at bit (i%eps)*bits, LSB-first,
+ *                    top 64-eps*bits bits of each word dead (zero when we
+ *                    write them) (get_element, src/common_code.cuh:43-50)
+ *   AM_FMT_BITSTREAM "nogaps": dense LSB-first bitstream, element i starts at
+ *                    bit i*bits (get_elem_nogaps, src/common_code.cuh:100-116)
+ *   AM_FMT_TIGHT     smallest of u8/u16/u32/u64 holding `bits`, masked
+ *                    (tight_type / sentinel, src/common_code.cuh:11-19,
+ *                    src/slab_iterator.cuh:83-91)
+ *   AM_FMT_U32       one uint32 per element (bits <= 32): the "uncompressed
+ *                    uint32" column of the benchmark configurations
+ * Every buffer is a whole number of uint64 words (am_packed_words); padding
+ * and dead bits written by this library are zero.
+ *
+ * Bitmasks are LSB-first: element i is bit (i%64) of uint64 word i/64, which
+ * on little-endian is bit (i%8) of byte i/8 -- the bit_iterator view
+ * (src/bit_iterator.cuh:18-21) and the u32 mask words the reference filters
+ * write (src/kernels/filter_slabs.cuh:55). Mask buffers hold ceil(n/64)
+ * words (src/main.cu:234); bits past n are written as zero.
+ */
+#ifndef AMETHYST_H
+#define AMETHYST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AM_ABI_VERSION 1
+
+typedef enum {
+    AM_OK = 0,
+    AM_E_INVALID = 1,            /* bad argument (null pointer, width, format) */
+    AM_E_UNSUPPORTED_CONFIG = 2, /* replaces the reference's -1 "skip" return */
+    AM_E_HIP = 3,                /* a HIP runtime call failed */
+    AM_E_WORKSPACE = 4,          /* workspace missing or too small */
+    AM_E_CAPACITY = 5            /* output buffer too small for the request */
+} am_status;
+
+typedef enum {
+    AM_FMT_BASELINE = 0,
+    AM_FMT_SLAB = 1,
+    AM_FMT_BITSTREAM = 2,
+    AM_FMT_TIGHT = 3,
+    AM_FMT_U32 = 4
+} am_format;
+
+/* A column: `n` elements of `bits` (1..64) bits in `format` at `data`. */
+typedef struct {
+    void* data;
+    uint64_t n;
+    uint32_t bits;
+    uint32_t format;
+} am_column;
+
+/* Filter predicates: element v is selected iff `v OP c`.
+ * The reference predicate FILTER_PREDICATE_OP<bits> (src/config.cuh:107-113)
+ * is {AM_PRED_GE, 1 << (bits-1)}. */
+typedef enum {
+    AM_PRED_LT = 0,
+    AM_PRED_LE = 1,
+    AM_PRED_GT = 2,
+    AM_PRED_GE = 3,
+    AM_PRED_EQ = 4,
+    AM_PRED_NE = 5
+} am_pred_op;
+
+typedef struct {
+    uint32_t op;
+    uint32_t reserved;
+    uint64_t c;
+} am_predicate;
+
+/* Binary operators, r = OP(l, r) & mask(out.bits) computed in uint64
+ * (src/config.cuh:5-42). AND/OR/XOR/SUB are bitwise/wrapping extensions.
+ * Division and modulo by zero follow the RISC-V convention: x/0 = all ones,
+ * x%0 = x (the reference leaves them undefined). */
+typedef enum {
+    AM_OP_ADD = 0,          /* op_add          src/config.cuh:5-9   */
+    AM_OP_MUL = 1,          /* op_mul          src/config.cuh:27-31 */
+    AM_OP_MOD = 2,          /* op_mod          src/config.cuh:17-21 */
+    AM_OP_DIV = 3,          /* op_div          src/config.cuh:22-26 */
+    AM_OP_MATH = 4,         /* op_math         src/config.cuh:11-15 */
+    AM_OP_BOOL_SIMPLE = 5,  /* op_bool_simple  src/config.cuh:32-36 (l || r) */
+    AM_OP_BOOL_COMPLEX = 6, /* op_bool_complex src/config.cuh:38-42 (l xor r, logical) */
+    AM_OP_AND = 7,
+    AM_OP_OR = 8,
+    AM_OP_XOR = 9,
+    AM_OP_SUB = 10
+} am_binop;
+
+typedef enum {
+    AM_GB_EXACT = 0, /* u64 SUM + u64 COUNT per key */
+    AM_GB_REF = 1    /* SUM mod 2^value_bits, the reference's semantics
+                        (src/kernels/groupby_sm_array.cuh:77) */
+} am_groupby_mode;
+
+/* ---------------------------------------------------------------- helpers */
+
+int am_abi_version(void);
+const char* am_status_string(int status);
+
+/* uint64 words a column of n elements occupies in `format` at `bits`
+ * (0 for an invalid combination). */
+uint64_t am_packed_words(uint32_t format, uint32_t bits, uint64_t n);
+
+/* Output-width rules op_add_outbits/op_mul_outbits/op_mod_outbits
+ * (src/config.cuh:44-55): add -> bits+1, mul -> 2*bits, both capped at 64;
+ * every other op keeps `bits`. */
+uint32_t am_outbits(uint32_t op, uint32_t bits);
+
+/* Smallest width that holds every value the predicate can select from a
+ * column of `bits` bits (e.g. v < 1024 -> 10). */
+uint32_t am_pred_width(am_predicate pred, uint32_t bits);
+
+/* gen_bitmask (src/common_code.cuh:38-41), defined for bits in 1..64. */
+uint64_t am_gen_bitmask(uint32_t bits);
+
+/* Workspace used by the single-pass operators (compressstore, sum,
+ * group-by). It must be zero-filled once before first use (am_workspace_init
+ * or hipMemset); the kernels leave it ready for the next call. Calls that
+ * share one workspace must be ordered on one stream. */
+uint64_t am_workspace_bytes(uint64_t max_elements);
+int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------- data generation */
+
+/* Fill out[0..n) with the fast_prng stream (PCG32 XSH-RR seeded through
+ * splitmix64, inc = 1; src/fast_prng.cuh:11-30) starting at draw `offset`,
+ * computed in parallel with PCG jump-ahead: out[i] = draw (offset + i). */
+int am_prng_fill_u32(uint64_t seed, uint64_t offset, uint32_t* out, uint64_t n, void* stream);
+
+/* Uniform w-bit values v_i = draw(i) & mask(bits) packed into `out`. */
+int am_gen_uniform(uint64_t seed, am_column* out, void* stream);
+
+/* ---------------------------------------------------------------- operators */
+
+/* Convert a column between formats/widths: out[i] = in[i] & mask(out.bits).
+ * Replaces the reference's host-side packing and its accessors
+ * get_element / get_elem_nogaps / get_element_baseline
+ * (src/common_code.cuh:43-55,100-116). out->n must equal in->n. */
+int am_convert(const am_column* in, am_column* out, void* stream);
+
+/* Predicate filter -> LSB-first bitmask (ceil(n/64) words).
+ * Replaces kernel_filter_baseline / kernel_filter_nogaps / kernel_filter_slabs
+ * (src/kernels/filter_baseline.cuh:9, filter_nogaps.cuh:4, filter_slabs.cuh:4)
+ * and the wrapper filter_baseline (src/lib.cuh:43-46). */
+int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_out, void* stream);
+
+/* Order-preserving compaction of `in` into `out` (any format, out->bits may
+ * differ from in->bits). Selection comes from `mask` (LSB-first, ceil(n/64)
+ * words) or, when mask == NULL, from `pred` evaluated on the fly (fused
+ * filter + compressstore). If mask_out != NULL the selection bitmask is
+ * written there too. out->n is the capacity in elements; writes beyond it
+ * are dropped. stats_dev (device, 2 x uint64) receives
+ * {selected count, number of selected values that did not fit out->bits};
+ * values are stored masked to out->bits.
+ * One launch, decoupled look-back: replaces kernel_compresstore_slabs_chunk_popcount
+ * + cub::DeviceScan::ExclusiveSum + kernel_compresstore_slabs_write
+ * (src/kernels/compressstore_slabs.cuh:4-206,
+ * src/benchs/bench_compressstore_slabs.cu:38-56) and the wrapper
+ * compressstore_baseline_cub (src/lib.cuh:24-39). */
+int am_compressstore(const am_column* in, const uint64_t* mask, const am_predicate* pred,
+                     am_column* out, uint64_t* mask_out, uint64_t* stats_dev,
+                     void* ws, uint64_t ws_bytes, void* stream);
+
+/* Element-wise out[i] = op(a[i], b[i]) & mask(out.bits); a and b share n and
+ * bits; any input format x any output format.
+ * Replaces kernel_binary_op_baseline / _nogaps / _slabs
+ * (src/kernels/binary_op_baseline.cuh:4, binary_op_nogaps.cuh:5,
+ * binary_op_slabs.cuh:7) and the wrappers in src/lib.cuh:4-20. */
+int am_binary(uint32_t op, const am_column* a, const am_column* b, am_column* out, void* stream);
+
+/* Wrapping uint64 sum of all elements into *sum_dev (device pointer).
+ * Replaces kernel_hop_slabs / kernel_hop_nogaps (src/kernels/hop.cuh:7,61)
+ * with HOP_INIT = 0, HOP_OP = + (src/config.cuh:63-71). */
+int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t ws_bytes, void* stream);
+
+/* Group-by over keys (bits <= 12) with values (bits <= 32). Rows whose value
+ * fails `value_pred` are skipped when value_pred != NULL (the chained
+ * filter -> group-by). Output is in ascending key order:
+ *   out_keys   : present keys, slab-packed at keys->bits
+ *   out_sums   : per-key sums, slab-packed at result[1] bits
+ *   out_counts : per-key counts, slab-packed at result[2] bits (may be NULL)
+ * Each output buffer must hold am_packed_words(SLAB, bits, 2^keys->bits)
+ * words for the widest possible bits (64). result_dev (device, 4 x uint64)
+ * receives {group count, sum bits, count bits, rows aggregated}. In
+ * AM_GB_EXACT the sums are exact uint64 and the widths are minimal
+ * (>= 1); in AM_GB_REF sums are mod 2^values->bits and stored at that width.
+ * Replaces kernel_groupby_sm_array_gather / _writeout
+ * (src/kernels/groupby_sm_array.cuh:4-81), whose output order is
+ * nondeterministic; ours is key-sorted. */
+int am_groupby(const am_column* keys, const am_column* values, const am_predicate* value_pred,
+               uint32_t mode, uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts,
+               uint64_t* result_dev, void* ws, uint64_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMETHYST_H */

@@ -1,0 +1,207 @@
+"""GPU parity: every operator through the C ABI vs the CPU oracle, bit-exact.
+
+Inputs are seeded numpy draws packed by the oracle; outputs are compared as
+packed words (decoded values AND dead/padding bits), counts and masks.
+Edge cases follow the reference's validation loops: ragged lengths around
+the 64-element mask word and the 4096-element tile, empty and full
+selections, every width class (1, sub-byte, byte, straddling, 32, >32, 64).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = [0, 1, 2, 3, 4]  # baseline, slab, bitstream, tight, u32
+WIDTHS = [1, 3, 4, 8, 11, 17, 24, 31, 32, 33, 47, 63, 64]
+SIZES = [1, 63, 64, 65, 4095, 4096, 4097, 20011]
+
+
+def fmt_ok(fmt, bits):
+    return not (fmt == 4 and bits > 32)
+
+
+def rand_vals(rng, n, bits):
+    hi = rng.integers(0, 2**32, size=n, dtype=np.uint64)
+    lo = rng.integers(0, 2**32, size=n, dtype=np.uint64)
+    v = (hi << np.uint64(32)) | lo
+    m = np.uint64((1 << bits) - 1) if bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    return v & m
+
+
+def upload(am, oracle, vals, bits, fmt):
+    words = oracle.pack(fmt, bits, vals)
+    return am.Column.from_host_words(words, len(vals), bits, fmt)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_prng_matches_reference_stream(am, oracle_mod):
+    for seed in (42, 0, 7):
+        for off in (0, 1, 1000):
+            say("prng", seed, off)
+            got = am.prng_fill(seed, 5000, offset=off).cpu().numpy().view(np.uint32)
+            exp = oracle_mod.prng(seed, 5000, skip=off)
+            assert np.array_equal(got, exp)
+
+
+def test_gen_uniform(am, oracle_mod):
+    for fmt in FORMATS:
+        for bits in (3, 11, 32, 40, 64):
+            if not fmt_ok(fmt, bits):
+                continue
+            n = 3001
+            say("gen_uniform", fmt, bits)
+            col = am.gen_uniform(42, n, bits, fmt)
+            if bits <= 32:
+                vals = oracle_mod.prng(42, n).astype(np.uint64) & np.uint64((1 << bits) - 1)
+            else:
+                d = oracle_mod.prng(42, 2 * n).astype(np.uint64)
+                v = (d[0::2] << np.uint64(32)) | d[1::2]
+                vals = v & (np.uint64((1 << bits) - 1) if bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF))
+            assert np.array_equal(col.host_words(), oracle_mod.pack(fmt, bits, vals))
+
+
+def test_convert_all_formats(am, oracle_mod):
+    rng = np.random.default_rng(1)
+    for bits in WIDTHS:
+        vals = rand_vals(rng, 5003, bits)
+        for fi in FORMATS:
+            if not fmt_ok(fi, bits):
+                continue
+            col = upload(am, oracle_mod, vals, bits, fi)
+            for fo in FORMATS:
+                for ob in sorted({bits, max(1, bits - 2), min(64, bits + 5)}):
+                    if not fmt_ok(fo, ob):
+                        continue
+                    say("convert", bits, fi, fo, ob)
+                    out = am.convert(col, ob, fo)
+                    exp = oracle_mod.pack(fo, ob, vals & np.uint64(oracle_mod.bitmask(ob)))
+                    assert np.array_equal(out.host_words(), exp), (bits, fi, fo, ob)
+
+
+def test_filter(am, oracle_mod):
+    rng = np.random.default_rng(2)
+    for bits in WIDTHS:
+        for n in (1, 64, 4097, 20011):
+            vals = rand_vals(rng, n, bits)
+            for fmt in (0, 1, 2, 4):
+                if not fmt_ok(fmt, bits):
+                    continue
+                col = upload(am, oracle_mod, vals, bits, fmt)
+                half = 1 << (bits - 1)
+                for op, c in ((3, half), (0, half), (4, int(vals[0])), (5, int(vals[0])), (1, 0), (2, 2**64 - 1)):
+                    say("filter", bits, n, fmt, op, c)
+                    got = am.u64(am.filter(col, op, c))[:(n + 63) // 64]
+                    exp = oracle_mod.filter(fmt, bits, oracle_mod.pack(fmt, bits, vals), n, op, c)
+                    assert np.array_equal(got, exp), (bits, n, fmt, op)
+
+
+def _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, mask_words=None, pred=None, cap=None, ws=None):
+    n = len(vals)
+    col = upload(am, oracle_mod, vals, bits, fi)
+    if pred is not None:
+        mask_words = oracle_mod.filter(fi, bits, oracle_mod.pack(fi, bits, vals), n, pred[0], pred[1])
+        out, stats, mo = am.compressstore(col, ob, fo, pred=pred, want_mask=True, capacity=cap, ws=ws)
+    else:
+        import torch
+        m = torch.from_numpy(np.ascontiguousarray(mask_words).view(np.int64)).cuda()
+        out, stats, mo = am.compressstore(col, ob, fo, mask=m, want_mask=True, capacity=cap, ws=ws)
+    k, exp_words, ovf = oracle_mod.compressstore(fi, bits, oracle_mod.pack(fi, bits, vals), n, mask_words, fo, ob,
+                                                 out_cap=cap)
+    st = am.u64(stats)
+    assert int(st[0]) == k
+    assert int(st[1]) == ovf
+    assert np.array_equal(am.u64(mo)[:(n + 63) // 64], mask_words)
+    # words the oracle defines: those holding elements < min(k, cap)
+    kk = min(k, out.n)
+    used = oracle_mod.packed_words(fo, ob, kk)
+    got = out.host_words()[:used]
+    assert np.array_equal(got, exp_words[:used]), (bits, fi, ob, fo, n)
+
+
+def test_compress_mask_formats(am, oracle_mod):
+    rng = np.random.default_rng(3)
+    ws = am.Workspace(1 << 16)
+    for bits in (1, 4, 11, 17, 32, 33, 64):
+        for n in SIZES:
+            vals = rand_vals(rng, n, bits)
+            for sel in (0.0, 0.01, 0.5, 1.0):
+                mbits = rng.random(n) < sel
+                mask = np.packbits(mbits, bitorder="little")
+                mask = np.concatenate([mask, np.zeros((-len(mask)) % 8, np.uint8)]).view(np.uint64)
+                for fi in (0, 1, 2):
+                    for fo, ob in ((1, bits), (2, bits), (0, bits), (4, min(bits, 32)), (1, max(1, bits - 1))):
+                        if not fmt_ok(fo, ob):
+                            continue
+                        say("compress-mask", bits, n, sel, fi, fo, ob)
+                        _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, mask_words=mask, ws=ws)
+
+
+def test_compress_predicate_fused(am, oracle_mod):
+    rng = np.random.default_rng(4)
+    ws = am.Workspace(1 << 20)
+    for bits in (4, 8, 11, 17, 24, 32):
+        n = 300007
+        vals = rand_vals(rng, n, bits)
+        for frac in (0.01, 0.5, 0.99):
+            c = int(round(frac * (1 << bits)))
+            pw = am.pred_width(0, c, bits)
+            for fo, ob in ((4, bits), (1, bits), (2, bits), (1, pw), (2, pw), (3, pw)):
+                say("compress-pred", bits, frac, fo, ob)
+                _check_compress(am, oracle_mod, vals, bits, 1, ob, fo, pred=(0, c), ws=ws)
+
+
+def test_compress_capacity_truncation(am, oracle_mod):
+    rng = np.random.default_rng(5)
+    ws = am.Workspace(1 << 16)
+    vals = rand_vals(rng, 50000, 11)
+    for cap in (0, 1, 100, 7777, 24000):
+        for fo, ob in ((1, 11), (2, 11), (4, 11)):
+            say("compress-cap", cap, fo)
+            _check_compress(am, oracle_mod, vals, 11, 1, ob, fo, pred=(0, 1024), cap=cap, ws=ws)
+
+
+def test_compress_workspace_reuse(am, oracle_mod):
+    """Many launches on one workspace with varying tile counts (parity reset)."""
+    rng = np.random.default_rng(6)
+    ws = am.Workspace(1 << 18)
+    for i, n in enumerate([200000, 1, 5000, 262144, 4096, 100, 150000, 9000]):
+        vals = rand_vals(rng, n, 11)
+        say("compress-reuse", i, n)
+        _check_compress(am, oracle_mod, vals, 11, 1, 11, 2, pred=(0, 700), ws=ws)
+
+
+def test_binary_ops(am, oracle_mod):
+    rng = np.random.default_rng(7)
+    for bits in (1, 4, 8, 11, 17, 24, 31, 32, 40, 63, 64):
+        n = 4099
+        a = rand_vals(rng, n, bits)
+        b = rand_vals(rng, n, bits)
+        b[::7] = 0  # exercise div/mod by zero
+        for fmt in (0, 1, 2, 3):
+            ca = upload(am, oracle_mod, a, bits, fmt)
+            cb = upload(am, oracle_mod, b, bits, fmt)
+            for op in range(11):
+                for ofmt in (fmt, 1 if fmt != 1 else 2):
+                    ob = oracle_mod.outbits(op, bits)
+                    say("binary", bits, fmt, op, ofmt, ob)
+                    out = am.binary(op, ca, cb, out_bits=ob, out_fmt=ofmt)
+                    exp = oracle_mod.binary(op, fmt, bits, oracle_mod.pack(fmt, bits, a), oracle_mod.pack(fmt, bits, b),
+                                            n, ofmt, ob)
+                    assert np.array_equal(out.host_words(), exp), (bits, fmt, op, ofmt)
+
+
+def test_sum(am, oracle_mod):
+    rng = np.random.default_rng(8)
+    ws = am.Workspace(0)
+    for bits in (1, 4, 11, 20, 32, 33, 64):
+        for n in (0, 1, 4095, 4097, 1000003):
+            vals = rand_vals(rng, n, bits)
+            for fmt in (0, 1, 2, 3):
+                say("sum", bits, n, fmt)
+                col = upload(am, oracle_mod, vals, bits, fmt)
+                got = int(am.u64(am.sum(col, ws=ws))[0])
+                exp = oracle_mod.sum(fmt, bits, oracle_mod.pack(fmt, bits, vals), n)
+                assert got == exp, (bits, n, fmt)
