@@ -44,6 +44,12 @@ class am_column(ctypes.Structure):
                 ("format", ctypes.c_uint32)]
 
 
+class am_matmul_desc(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("C_packed", ctypes.c_void_p),
+                ("C_i32", ctypes.c_void_p), ("M", ctypes.c_uint32), ("N", ctypes.c_uint32), ("K", ctypes.c_uint32),
+                ("out_bits", ctypes.c_uint32)]
+
+
 class am_predicate(ctypes.Structure):
     _fields_ = [("op", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("c", ctypes.c_uint64)]
 
@@ -86,6 +92,11 @@ def lib():
             "am_compressstore": ([colp, vp, predp, colp, vp, vp, vp, u64, vp], i32),
             "am_binary": ([u32, colp, colp, colp, vp], i32),
             "am_sum": ([colp, vp, vp, u64, vp], i32),
+            "am_groupby": ([colp, colp, predp, u32, vp, vp, vp, vp, vp, u64, vp], i32),
+            "am_matmul_out_bits": ([u32, u32], u32),
+            "am_matmul_workspace_bytes": ([vp, vp, vp, u32], u64),
+            "am_matmul_packed": ([u32, vp, vp, u32, u32, u32, u32, vp, vp, vp, u64, vp], i32),
+            "am_matmul_packed_batched": ([u32, vp, u32, vp, u64, vp], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
@@ -288,6 +299,63 @@ def sum(col, ws=None, out=None, stream=None):
     _check(lib().am_sum(ctypes.byref(a), ctypes.c_void_p(out.data_ptr()), ws.ptr(), ws.bytes, _stream(stream)),
            "am_sum")
     return out
+
+
+def groupby(keys, values, mode=GB_EXACT, value_pred=None, want_counts=True, ws=None, stream=None):
+    """Key-ordered group-by; returns (out_keys, out_sums, out_counts, result) device tensors.
+
+    result = {groups, sum bits, count bits, rows aggregated}; the outputs are
+    slab-packed at keys.bits / result[1] / result[2].
+    """
+    g = 1 << keys.bits
+    cap = packed_words(SLAB, 1, g)  # enough for any width (1 bit: 64 per word ... 64 bits: g words)
+    cap = max(cap, g)
+    ok, osum, ocnt = device_words(cap), device_words(cap), device_words(cap) if want_counts else None
+    res = device_words(4)
+    ws = ws if ws is not None else Workspace(0)
+    ck, cv = keys.c(), values.c()
+    pp = ctypes.byref(am_predicate(value_pred[0], 0, value_pred[1])) if value_pred is not None else None
+    _check(lib().am_groupby(ctypes.byref(ck), ctypes.byref(cv), pp, mode, ctypes.c_void_p(ok.data_ptr()),
+                            ctypes.c_void_p(osum.data_ptr()),
+                            ctypes.c_void_p(ocnt.data_ptr()) if ocnt is not None else None,
+                            ctypes.c_void_p(res.data_ptr()), ws.ptr(), ws.bytes, _stream(stream)), "am_groupby")
+    return ok, osum, ocnt, res
+
+
+def matmul_out_bits(bits, K):
+    return int(lib().am_matmul_out_bits(bits, K))
+
+
+def matrix_words(rows, cols, bits):
+    """Words of a row-padded slab matrix (each row padded to whole slabs)."""
+    eps = 64 // bits
+    return rows * ((cols + eps - 1) // eps)
+
+
+def matmul_packed_batched(bits, problems, want_i32=False, out_bits=None, stream=None):
+    """problems: list of (A_words_tensor, B_words_tensor, M, N, K).
+
+    Returns a list of (C_packed tensor, C_i32 tensor or None, out_bits)."""
+    torch = _torch()
+    n = len(problems)
+    Ms = (ctypes.c_uint32 * n)(*[p[2] for p in problems])
+    Ns = (ctypes.c_uint32 * n)(*[p[3] for p in problems])
+    Ks = (ctypes.c_uint32 * n)(*[p[4] for p in problems])
+    wsb = int(lib().am_matmul_workspace_bytes(Ms, Ns, Ks, n))
+    ws = device_words((wsb + 7) // 8)
+    descs = (am_matmul_desc * n)()
+    outs = []
+    for i, (A, B, M, N, K) in enumerate(problems):
+        ob = out_bits if out_bits else matmul_out_bits(bits, K)
+        cp = device_words(matrix_words(M, N, ob))
+        ci = torch.zeros(M * N, dtype=torch.int32, device="cuda") if want_i32 else None
+        descs[i] = am_matmul_desc(A.data_ptr(), B.data_ptr(), cp.data_ptr(), ci.data_ptr() if ci is not None else None,
+                                  M, N, K, ob)
+        outs.append((cp, ci, ob))
+    _check(lib().am_matmul_packed_batched(bits, descs, n, ctypes.c_void_p(ws.data_ptr()), wsb, _stream(stream)),
+           "am_matmul_packed_batched")
+    outs.append(ws)  # keep the workspace alive until the caller synchronises
+    return outs
 
 
 def u64(t):

@@ -104,6 +104,46 @@ struct TileReader {
     }
 };
 
+// Raw fetch of one element for the lane: the loaded word(s) plus the shift
+// that extracts the value, so loads can be issued a whole tile ahead.
+template <int IK, bool WIDE>
+struct Fetch {
+    uint64_t raw;
+    uint32_t sh;
+    __device__ inline void load(const TileReader<IK>& rd, uint32_t i, bool inb)
+    {
+        raw = 0;
+        sh = 0;
+        if (!inb) return;
+        if (IK == K_STRIDED) {
+            switch (rd.cbytes) {
+                case 1: raw = ((const uint8_t*)rd.base)[i]; break;
+                case 2: raw = ((const uint16_t*)rd.base)[i]; break;
+                case 4: raw = ((const uint32_t*)rd.base)[i]; break;
+                default: raw = rd.base[i];
+            }
+        }
+        else if (IK == K_SLAB) {
+            const uint32_t idx = rd.r0 + i;
+            const uint32_t q = rd.eps == 1 ? idx : __umulhi(idx, rd.eps_magic);
+            raw = rd.base[q];
+            sh = (idx - q * rd.eps) * rd.bits;
+        }
+        else if (!WIDE) {
+            const uint32_t lb = rd.r0 + i * rd.bits;
+            const uint32_t* p32 = (const uint32_t*)rd.base;
+            const uint32_t lo = p32[lb >> 5];
+            const uint32_t hi = ((lb & 31) + rd.bits > 32) ? p32[(lb >> 5) + 1] : 0u;
+            raw = ((uint64_t)hi << 32) | lo;
+            sh = lb & 31;
+        }
+        else {
+            raw = rd.get(i);
+        }
+    }
+    __device__ inline uint64_t value(uint64_t mask) const { return (raw >> sh) & mask; }
+};
+
 // ------------------------------------------------------------ wave64 helpers
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
@@ -187,60 +227,87 @@ __device__ inline void drain() { __builtin_amdgcn_s_waitcnt(0); }
 
 // ------------------------------------------------------------ workspace
 
-// Workspace header; followed by the group-by table and the look-back
-// descriptors. Every kernel that uses it returns the counters to zero.
+// Workspace: header, a small scratch region (sum partials, group-by totals),
+// the group-by partial tables, then the look-back descriptors.
 struct WsHeader {
-    uint32_t ticket;   // dynamic tile id counter
-    uint32_t done;     // finished-block counter
-    uint32_t epoch;    // launch counter of the look-back kernels (parity)
-    uint32_t lastT[2]; // tiles used by the last look-back launch of each parity
-    uint32_t pad0[3];
-    uint64_t overflow; // compaction: values wider than the output width
-    uint64_t accum;    // sum
-    uint64_t rows;     // group-by rows aggregated
-    uint64_t pad1[25];
+    uint32_t epoch; // launch counter of the look-back kernels
+    uint32_t pad0[63];
 };
 static_assert(sizeof(WsHeader) == 256, "header size");
 
-constexpr uint64_t WS_GB_ENTRIES = 4096; // group-by table, keys <= 12 bits
-constexpr uint64_t WS_GB_BYTES = WS_GB_ENTRIES * 16;
-constexpr uint64_t WS_FIXED = 256 + WS_GB_BYTES;
-constexpr uint64_t WS_DESC_BYTES_PER_TILE = 6 * 8; // 3 arrays x 2 parities
+constexpr uint64_t WS_SMALL_BYTES = 65536;                  // 8192 u64
+constexpr uint32_t GB_MAX_BLOCKS = 256;                      // group-by partial tables
+constexpr uint64_t GB_MAX_KEYS = 4096;                       // keys <= 12 bits
+constexpr uint64_t WS_GBPART_BYTES = GB_MAX_BLOCKS * 2 * GB_MAX_KEYS * 8; // 16 MiB
+constexpr uint64_t WS_FIXED = 256 + WS_SMALL_BYTES + WS_GBPART_BYTES;
+constexpr uint64_t WS_DESC_BYTES_PER_TILE = 5 * 8; // status + 2 window words per level
 
+__host__ __device__ inline uint64_t* ws_small(void* ws) { return (uint64_t*)((uint8_t*)ws + 256); }
+__host__ __device__ inline uint64_t* ws_gbpart(void* ws) { return (uint64_t*)((uint8_t*)ws + 256 + WS_SMALL_BYTES); }
+
+// Look-back descriptors. Every word carries the launch epoch, so stale words
+// from earlier launches are recognised without ever clearing the arrays:
+//   status[t]    [63:62] level (1 aggregate, 2 inclusive) [61:40] epoch&(2^22-1) [39:0] count
+//   winA/winP[2t..2t+1]  [63:48] epoch&0xffff [47:46] level [45:0] payload
+// The two payload words hold a 63-bit window (46 + 17 bits) and a 29-bit
+// saturating overflow count. Each launch also zeroes one slice of the
+// indices it does not use, so no word can outlive 2^16 launches unrewritten.
 struct Lookback {
-    uint64_t* status; // [2][cap]  bits 63..62 state (1 = aggregate, 2 = inclusive), 61..0 count
-    uint64_t* tailA;  // [2][cap]  bit 63 valid, bits 62..0 window of the tile's own values
-    uint64_t* tailP;  // [2][cap]  bit 63 valid, bits 62..0 window of the inclusive prefix
+    uint64_t* status; // [cap]
+    uint64_t* winA;   // [2*cap]
+    uint64_t* winP;   // [2*cap]
     uint64_t cap;
 };
 
 __host__ __device__ inline Lookback lookback_of(void* ws, uint64_t ws_bytes)
 {
     Lookback L;
-    uint64_t cap = (ws_bytes - WS_FIXED) / WS_DESC_BYTES_PER_TILE;
+    uint64_t cap = ws_bytes > WS_FIXED ? (ws_bytes - WS_FIXED) / WS_DESC_BYTES_PER_TILE : 0;
     uint64_t* d = (uint64_t*)((uint8_t*)ws + WS_FIXED);
     L.status = d;
-    L.tailA = d + 2 * cap;
-    L.tailP = d + 4 * cap;
+    L.winA = d + cap;
+    L.winP = d + 3 * cap;
     L.cap = cap;
     return L;
 }
 
-constexpr uint64_t ST_A = 1ull << 62;
-constexpr uint64_t ST_P = 2ull << 62;
-constexpr uint64_t ST_COUNT = (1ull << 62) - 1;
-constexpr uint64_t TAIL_VALID = 1ull << 63;
-constexpr uint64_t TAIL_BITS = (1ull << 63) - 1;
+constexpr uint32_t LB_CLEAR_SLICES = 64;
+constexpr uint64_t CNT_MASK = (1ull << 40) - 1;
+constexpr uint64_t P46 = (1ull << 46) - 1;
+constexpr uint64_t OVF_MAX = (1ull << 29) - 1;
+constexpr uint64_t WIN_BITS = (1ull << 63) - 1;
 
-// Scan element of the compaction: (count, last-63-bits window of the dense
-// stream of selected values at the output width). Windows are right-aligned:
-// bit 62 is the last stream bit. Appending y (m = y.count*bits stream bits)
-// to x gives (x.win >> m) | y.win when m < 63, else y.win.
+__device__ inline uint64_t lb_status(uint32_t level, uint32_t epoch, uint64_t count)
+{
+    return ((uint64_t)level << 62) | ((uint64_t)(epoch & 0x3FFFFF) << 40) | (count & CNT_MASK);
+}
+__device__ inline bool lb_status_ok(uint64_t s, uint32_t epoch)
+{
+    return (s >> 62) != 0 && ((s >> 40) & 0x3FFFFF) == (epoch & 0x3FFFFF);
+}
+__device__ inline uint64_t lb_word(uint32_t level, uint32_t epoch, uint64_t payload)
+{
+    return ((uint64_t)(epoch & 0xFFFF) << 48) | ((uint64_t)level << 46) | (payload & P46);
+}
+__device__ inline bool lb_word_ok(uint64_t w, uint32_t level, uint32_t epoch)
+{
+    return (w >> 48) == (epoch & 0xFFFF) && ((w >> 46) & 3) == level;
+}
+
+// Scan element of the compaction: (count, window, overflow). The window is
+// the last 63 bits of the dense stream of selected values at the output
+// width, right-aligned (bit 62 = last stream bit). Appending y
+// (m = y.count*bits stream bits) to x: (x.win >> m) | y.win if m < 63, else y.win.
 __device__ inline uint64_t win_combine(uint64_t xw, uint64_t yw, uint64_t ycount, uint32_t bits)
 {
     uint64_t m = ycount * bits;
-    if (m >= 63) return yw & TAIL_BITS;
-    return ((xw & TAIL_BITS) >> m) | (yw & TAIL_BITS);
+    if (m >= 63) return yw & WIN_BITS;
+    return ((xw & WIN_BITS) >> m) | (yw & WIN_BITS);
+}
+__device__ inline uint64_t ovf_add(uint64_t a, uint64_t b)
+{
+    uint64_t s = a + b;
+    return s > OVF_MAX ? OVF_MAX : s;
 }
 
 } // namespace am

@@ -54,4 +54,17 @@ inline int check_launch()
     return e == hipSuccess ? AM_OK : AM_E_HIP;
 }
 
+// Largest grid whose blocks are all resident at once (the tiles of a block
+// wait only on lower tiles, which then always have a running owner).
+template <typename K>
+static uint32_t resident_blocks(K kernel, int threads, size_t lds)
+{
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds) != hipSuccess || per < 1) per = 1;
+    if (cus < 1) cus = 1;
+    return (uint32_t)(per * cus);
+}
+
 } // namespace am

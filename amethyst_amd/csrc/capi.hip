@@ -79,7 +79,8 @@ extern "C" uint64_t am_gen_bitmask(uint32_t bits) { return bits == 0 ? 0 : lowma
 
 extern "C" uint64_t am_workspace_bytes(uint64_t max_elements)
 {
-    // one descriptor set per 4096-element tile (the compaction tile), plus one
+    // fixed part (sum partials, group-by tables) + one descriptor set per
+    // 4096-element compaction tile
     uint64_t tiles = (max_elements + 4095) / 4096 + 1;
     return WS_FIXED + tiles * WS_DESC_BYTES_PER_TILE;
 }

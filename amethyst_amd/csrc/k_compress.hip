@@ -7,26 +7,28 @@
 // re-reads an input word from global memory for every output element, and
 // merges the slab words shared by two warps with atomicAnd/atomicOr.
 //
-// Here it is one launch. Each 256-thread block takes a 4096-element tile
-// from a ticket counter (tiles start in ticket order, so look-back always
-// makes progress), and:
+// Here it is one launch and it uses no contended global atomics. A grid of
+// at most as many 256-thread blocks as can be resident at once walks the
+// 4096-element tiles in order (block b takes tiles b, b+G, ...; a tile only
+// waits on lower tiles, whose blocks are running, so look-back always makes
+// progress). Each block loads the next tile while it works on the current
+// one, and per tile:
 //   1. decodes element base + 64j + lane per lane; the selection flag (mask
 //      bit or the predicate evaluated in place) goes through one wave64
 //      ballot, which is also the output mask word and, via v_mbcnt, each
 //      lane's rank among the wave's selected elements;
 //   2. publishes its aggregate and walks back over predecessors 64 at a
-//      time (one descriptor per lane), combining (count, window) pairs with
-//      an ordered butterfly; the window is the last 63 bits of the packed
-//      output stream, so the tile that completes an output word can build it
-//      from its predecessors' values without atomics;
+//      time (one descriptor per lane), combining (count, window, overflow)
+//      with an ordered butterfly; the window is the last 63 bits of the
+//      packed output stream, so the tile that completes an output word can
+//      build it from its predecessors' values without atomics;
 //   3. writes its selected values: byte-aligned outputs (u8/u16/u32/u64
 //      containers) are scattered directly at their rank; slab and bitstream
 //      outputs are assembled one whole word per lane from the LDS-compacted
 //      tile plus the carried window, so every word is written exactly once,
 //      dead and padding bits zero, no read-modify-write.
-// The descriptors live in a caller workspace; each launch clears the other
-// parity's descriptors and the last block to finish resets the counters, so
-// repeated launches (and graph replays) need no memset.
+// Descriptor words carry the launch epoch (bumped by the last tile), so the
+// workspace needs no clearing between launches or graph replays.
 #include "am_device.hpp"
 #include "am_host.hpp"
 
@@ -41,7 +43,7 @@ struct CompressArgs {
     const uint64_t* mask_in;
     uint64_t pc;
     uint32_t pop;
-    uint32_t T; // tiles in this launch
+    uint32_t T; // tiles
     Col out;
     uint64_t cap; // output capacity in elements
     uint64_t* mask_out;
@@ -60,255 +62,268 @@ __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
     }
 }
 
-template <int IK, int OK, bool PRED>
+template <int IK, int OK, bool PRED, bool WIDE>
 __global__ __launch_bounds__(256) void k_compress(CompressArgs a)
 {
-    __shared__ uint32_t s_tile, s_par, s_lastT;
     __shared__ uint32_t s_wcnt[CS_WAVES];
+    __shared__ uint32_t s_wovf[CS_WAVES];
     __shared__ uint64_t s_P, s_carry;
+    __shared__ uint32_t s_epoch;
     __shared__ uint64_t s_vals[OK == K_STRIDED ? 1 : CS_TILE];
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t bo = a.out.bits;
     const uint64_t om = a.out.mask;
-
-    if (tid == 0) {
-        s_tile = add32(&a.hdr->ticket, 1);
-        uint32_t ep = ld32(&a.hdr->epoch);
-        s_par = ep & 1;
-        s_lastT = ld32(&a.hdr->lastT[(ep & 1) ^ 1]);
-    }
-    __syncthreads();
-    const uint32_t t = s_tile, par = s_par;
-    const uint64_t cap = a.lb.cap;
-    uint64_t* st = a.lb.status + par * cap;
-    uint64_t* tA = a.lb.tailA + par * cap;
-    uint64_t* tP = a.lb.tailP + par * cap;
-    {
-        // clear the other parity's descriptors used by the previous launch
-        uint64_t* ost = a.lb.status + (par ^ 1) * cap;
-        uint64_t* otA = a.lb.tailA + (par ^ 1) * cap;
-        uint64_t* otP = a.lb.tailP + (par ^ 1) * cap;
-        for (uint64_t i = t + (uint64_t)tid * a.T; i < s_lastT; i += 256ull * a.T) {
-            st_rel(&ost[i], 0);
-            st_rel(&otA[i], 0);
-            st_rel(&otP[i], 0);
-        }
-    }
-
-    // ---- 1. decode, select, rank within the wave
     const uint64_t n = a.in.n;
-    const uint64_t wbase = (uint64_t)t * CS_TILE + wave * (CS_J * 64);
-    TileReader<IK> rd(a.in, wbase < n ? wbase : 0);
-    uint64_t v[CS_J];
-    uint64_t bal[CS_J];
-    uint32_t cnt = 0, ovf = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < CS_J; j++) {
-        const uint64_t e = wbase + j * 64 + lane;
-        const bool inb = e < n;
-        v[j] = inb ? rd.get(j * 64 + lane) : 0;
-        bool f;
-        if (PRED) {
-            f = inb && pred_eval(a.pop, a.pc, v[j]);
-        }
-        else {
-            const uint64_t mw = (wbase + j * 64 < n) ? a.mask_in[(wbase >> 6) + j] : 0;
-            f = inb && ((mw >> lane) & 1);
-        }
-        bal[j] = ballot(f);
-        cnt += (uint32_t)__popcll(bal[j]);
-        ovf += (uint32_t)__popcll(ballot(f && (v[j] & ~om) != 0));
-    }
-    if (a.mask_out) {
-        uint64_t mine = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < CS_J; j++)
-            if (lane == j) mine = bal[j];
-        const uint64_t w = (wbase >> 6) + lane;
-        if (lane < CS_J && w < ((n + 63) >> 6)) a.mask_out[w] = mine;
-    }
-    if (lane == 0) s_wcnt[wave] = cnt;
-    __syncthreads();
-    uint32_t wpre = 0, ctile = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < CS_WAVES; w++) {
-        uint32_t c = s_wcnt[w];
-        wpre += w < wave ? c : 0;
-        ctile += c;
-    }
+    const uint32_t T = a.T;
 
-    if (OK != K_STRIDED) {
-        uint32_t run = wpre;
+    if (tid == 0) s_epoch = ld32(&a.hdr->epoch);
+    __syncthreads();
+    const uint32_t epoch = s_epoch;
+    {
+        // zero one slice of the descriptors this launch does not use
+        const uint64_t cap = a.lb.cap;
+        const uint64_t per = (cap + LB_CLEAR_SLICES - 1) / LB_CLEAR_SLICES;
+        uint64_t lo = (uint64_t)(epoch % LB_CLEAR_SLICES) * per;
+        uint64_t hi = lo + per < cap ? lo + per : cap;
+        if (lo < T) lo = T;
+        for (uint64_t i = lo + (uint64_t)blockIdx.x * 256 + tid; i < hi; i += (uint64_t)gridDim.x * 256) {
+            st_rel(&a.lb.status[i], 0);
+            st_rel(&a.lb.winA[2 * i], 0);
+            st_rel(&a.lb.winA[2 * i + 1], 0);
+            st_rel(&a.lb.winP[2 * i], 0);
+            st_rel(&a.lb.winP[2 * i + 1], 0);
+        }
+    }
+    uint32_t t = blockIdx.x;
+    if (t >= T) return;
+
+    Fetch<IK, WIDE> f[CS_J];
+    uint64_t mk = 0; // lane j < CS_J: mask word j of this wave's chunk (mask mode)
+    auto fetch_tile = [&](uint32_t tt, Fetch<IK, WIDE>* ff, uint64_t& mm) {
+        const uint64_t wb = (uint64_t)tt * CS_TILE + wave * (CS_J * 64);
+        TileReader<IK> rd(a.in, wb < n ? wb : 0);
+#pragma unroll
+        for (uint32_t j = 0; j < CS_J; j++) ff[j].load(rd, j * 64 + lane, wb + j * 64 + lane < n);
+        if (!PRED) {
+            const uint64_t w = (wb >> 6) + lane;
+            mm = (lane < CS_J && w < ((n + 63) >> 6)) ? a.mask_in[w] : 0;
+        }
+    };
+    fetch_tile(t, f, mk);
+
+    for (; t < T; t += gridDim.x) {
+        // ---- prefetch the next tile of this block
+        Fetch<IK, WIDE> nf[CS_J];
+        uint64_t nmk = 0;
+        const uint32_t tn = t + gridDim.x;
+        if (tn < T) fetch_tile(tn, nf, nmk);
+
+        // ---- 1. decode, select, count (lane j keeps ballot j in `mine`)
+        const uint64_t wbase = (uint64_t)t * CS_TILE + wave * (CS_J * 64);
+        uint64_t mine = 0;
+        uint32_t cnt = 0, ovf = 0;
 #pragma unroll
         for (uint32_t j = 0; j < CS_J; j++) {
-            if ((bal[j] >> lane) & 1) s_vals[run + mbcnt(bal[j])] = v[j] & om;
-            run += (uint32_t)__popcll(bal[j]);
+            const bool inb = wbase + j * 64 + lane < n;
+            const uint64_t v = f[j].value(a.in.mask);
+            bool fl;
+            if (PRED) {
+                fl = inb && pred_eval(a.pop, a.pc, v);
+            }
+            else {
+                const uint64_t mw = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mk >> 32), j) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((uint32_t)mk, j);
+                fl = inb && ((mw >> lane) & 1);
+            }
+            const uint64_t b = ballot(fl);
+            if (lane == j) mine = b;
+            cnt += (uint32_t)__popcll(b);
+            ovf += (uint32_t)__popcll(ballot(fl && (v & ~om) != 0));
         }
-        __syncthreads();
-    }
-
-    // ---- 2. look-back (wave 0)
-    if (wave == 0) {
-        // own window: last 63 bits of this tile's packed stream
-        uint64_t own_w = 0;
-        if (OK != K_STRIDED && bo < 64) {
-            const uint32_t k = (63 + bo - 1) / bo;
-            const uint32_t take = ctile < k ? ctile : k;
-            if (lane < take) {
-                const uint32_t jx = ctile - take + lane;
-                const int64_t shift = (int64_t)jx * bo - ((int64_t)ctile * bo - 63);
-                const uint64_t val = s_vals[jx];
-                own_w = shift >= 0 ? (val << shift) : (val >> (-shift));
-            }
-            own_w = wave_or_u64(own_w) & TAIL_BITS;
-        }
-        uint64_t excl_c = 0, excl_w = 0;
-        if (t == 0) {
-            if (lane == 0) {
-                st_rel(&tP[0], TAIL_VALID | own_w);
-                st_rel(&st[0], ST_P | ctile);
-            }
-        }
-        else {
-            if (lane == 0) {
-                st_rel(&tA[t], TAIL_VALID | own_w);
-                st_rel(&st[t], ST_A | ctile);
-            }
-            uint64_t acc_c = 0, acc_w = 0;
-            int64_t pos = (int64_t)t - 1;
-            while (true) {
-                const int64_t idx = pos - (int64_t)lane;
-                uint64_t s = ST_P;
-                if (idx >= 0) {
-                    while ((s = ld_acq(&st[idx])) == 0) __builtin_amdgcn_s_sleep(1);
-                }
-                const bool isP = (s >> 62) == 2;
-                const uint64_t pm = ballot(isP);
-                const uint32_t stop = pm ? (uint32_t)__builtin_ctzll(pm) : 64;
-                const bool incl = lane <= stop;
-                uint64_t c = incl ? (s & ST_COUNT) : 0;
-                uint64_t w = 0;
-                if (incl && idx >= 0) {
-                    const uint64_t* tp = isP ? &tP[idx] : &tA[idx];
-                    uint64_t x;
-                    while (!((x = ld_acq(tp)) & TAIL_VALID)) __builtin_amdgcn_s_sleep(1);
-                    w = x & TAIL_BITS;
-                }
-                // ordered butterfly: higher lanes hold older tiles
-                for (int m = 1; m < 64; m <<= 1) {
-                    const uint64_t oc = shfl_xor_u64(c, m), ow = shfl_xor_u64(w, m);
-                    if (lane & m) w = win_combine(w, ow, oc, bo);
-                    else w = win_combine(ow, w, c, bo);
-                    c += oc;
-                }
-                acc_w = win_combine(w, acc_w, acc_c, bo);
-                acc_c += c;
-                if (pm) break;
-                pos -= 64;
-            }
-            excl_c = acc_c;
-            excl_w = acc_w;
-            if (lane == 0) {
-                st_rel(&tP[t], TAIL_VALID | win_combine(excl_w, own_w, ctile, bo));
-                st_rel(&st[t], ST_P | (excl_c + ctile));
-            }
+        if (a.mask_out) {
+            const uint64_t w = (wbase >> 6) + lane;
+            if (lane < CS_J && w < ((n + 63) >> 6)) a.mask_out[w] = mine;
         }
         if (lane == 0) {
-            s_P = excl_c;
-            s_carry = excl_w;
+            s_wcnt[wave] = cnt;
+            s_wovf[wave] = ovf;
         }
-    }
-    __syncthreads();
-    const uint64_t P = s_P, carry = s_carry;
-    const bool last = (t == a.T - 1);
-    const uint64_t ocap = a.cap;
-
-    // ---- 3. write
-    if (OK == K_STRIDED) {
-        uint32_t run = wpre;
+        __syncthreads();
+        uint32_t wpre = 0, ctile = 0, otile = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < CS_J; j++) {
-            if ((bal[j] >> lane) & 1) {
-                const uint64_t r = P + run + mbcnt(bal[j]);
-                if (r < ocap) store_container(a.out, r, v[j]);
-            }
-            run += (uint32_t)__popcll(bal[j]);
+        for (uint32_t w = 0; w < CS_WAVES; w++) {
+            const uint32_t c = s_wcnt[w];
+            wpre += w < wave ? c : 0;
+            ctile += c;
+            otile += s_wovf[w];
         }
-        if (last && tid == 0) {
-            // zero the padding bytes of the final word
-            uint64_t K = P + ctile;
-            if (K > ocap) K = ocap;
-            const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
-            for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
-        }
-    }
-    else if (OK == K_SLAB) {
-        const uint64_t eps = a.out.eps;
-        const uint64_t kf = P / eps;
-        uint64_t kl = (P + ctile) / eps;
-        if (last && ((P + ctile) % eps)) kl += 1;
-        const uint64_t wcap = (ocap + eps - 1) / eps;
-        if (kl > wcap) kl = wcap;
-        for (uint64_t k = kf + tid; k < kl; k += 256) {
-            uint64_t word = 0;
-            const uint64_t e0 = k * eps;
-            for (uint32_t s = 0; s < eps; s++) {
-                const uint64_t e = e0 + s;
-                if (e >= ocap) break;
-                uint64_t val;
-                if (e < P) val = (carry >> (63 - (P - e) * bo)) & om;
-                else if (e < P + ctile) val = s_vals[e - P];
-                else break;
-                word |= val << (s * bo);
-            }
-            a.out.p[k] = word;
-        }
-    }
-    else {
-        const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
-        const uint64_t kf = B0 >> 6;
-        uint64_t kl = B1 >> 6;
-        if (last && (B1 & 63)) kl += 1;
-        const uint64_t wcap = (Bcap + 63) >> 6;
-        if (kl > wcap) kl = wcap;
-        for (uint64_t k = kf + tid; k < kl; k += 256) {
-            const uint64_t b0 = k * 64;
-            uint64_t word = 0;
-            if (b0 < B0) word = carry >> (b0 + 63 - B0);
-            uint64_t e0 = b0 / bo;
-            if (e0 < P) e0 = P;
-            uint64_t e1 = (b0 + 63) / bo + 1;
-            if (e1 > P + ctile) e1 = P + ctile;
-            for (uint64_t e = e0; e < e1; e++) {
-                const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
-                const uint64_t val = s_vals[e - P];
-                word |= pos >= 0 ? (val << pos) : (val >> (-pos));
-            }
-            if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-            a.out.p[k] = word;
-        }
-    }
+        auto bal = [&](uint32_t j) -> uint64_t {
+            return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
+                   (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
+        };
 
-    // ---- finish: overflow count, last-block bookkeeping
-    if (lane == 0 && ovf) (void)add64(&a.hdr->overflow, ovf);
-    drain();
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t d = add32(&a.hdr->done, 1);
-        if (d == a.T - 1) {
-            uint64_t s;
-            while (((s = ld_acq(&st[a.T - 1])) >> 62) != 2) __builtin_amdgcn_s_sleep(1);
-            const uint64_t ov = __hip_atomic_exchange(&a.hdr->overflow, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (a.stats) {
-                a.stats[0] = s & ST_COUNT;
-                a.stats[1] = ov;
+        if (OK != K_STRIDED) {
+            uint32_t run = wpre;
+#pragma unroll
+            for (uint32_t j = 0; j < CS_J; j++) {
+                const uint64_t b = bal(j);
+                if ((b >> lane) & 1) s_vals[run + mbcnt(b)] = f[j].value(a.in.mask) & om;
+                run += (uint32_t)__popcll(b);
             }
-            st32(&a.hdr->lastT[par], a.T);
-            st32(&a.hdr->ticket, 0);
-            st32(&a.hdr->done, 0);
-            st32(&a.hdr->epoch, ld32(&a.hdr->epoch) + 1);
+            __syncthreads();
         }
+
+        // ---- 2. look-back (wave 0)
+        if (wave == 0) {
+            uint64_t own_w = 0;
+            if (OK != K_STRIDED && bo < 64) {
+                const uint32_t k = (63 + bo - 1) / bo;
+                const uint32_t take = ctile < k ? ctile : k;
+                if (lane < take) {
+                    const uint32_t jx = ctile - take + lane;
+                    const int64_t shift = (int64_t)jx * bo - ((int64_t)ctile * bo - 63);
+                    const uint64_t val = s_vals[jx];
+                    own_w = shift >= 0 ? (val << shift) : (val >> (-shift));
+                }
+                own_w = wave_or_u64(own_w) & WIN_BITS;
+            }
+            uint64_t excl_c = 0, excl_w = 0, excl_o = 0;
+            if (t > 0) {
+                if (lane == 0) {
+                    st_rel(&a.lb.winA[2 * t], lb_word(1, epoch, own_w));
+                    st_rel(&a.lb.winA[2 * t + 1], lb_word(1, epoch, (own_w >> 46) | ((uint64_t)otile << 17)));
+                    st_rel(&a.lb.status[t], lb_status(1, epoch, ctile));
+                }
+                int64_t pos = (int64_t)t - 1;
+                while (true) {
+                    const int64_t idx = pos - (int64_t)lane;
+                    uint64_t s = 0;
+                    uint32_t lvl = 2;
+                    if (idx >= 0) {
+                        while (!lb_status_ok(s = ld_acq(&a.lb.status[idx]), epoch)) __builtin_amdgcn_s_sleep(1);
+                        lvl = (uint32_t)(s >> 62);
+                    }
+                    const uint64_t pm = ballot(lvl == 2);
+                    const uint32_t stop = pm ? (uint32_t)__builtin_ctzll(pm) : 64;
+                    const bool incl = lane <= stop && idx >= 0;
+                    uint64_t c = 0, w = 0, o = 0;
+                    if (incl) {
+                        c = s & CNT_MASK;
+                        const uint64_t* wp = (lvl == 2 ? a.lb.winP : a.lb.winA) + 2 * idx;
+                        uint64_t w0, w1;
+                        while (!lb_word_ok(w0 = ld_acq(wp), lvl, epoch)) __builtin_amdgcn_s_sleep(1);
+                        while (!lb_word_ok(w1 = ld_acq(wp + 1), lvl, epoch)) __builtin_amdgcn_s_sleep(1);
+                        w = (w0 & P46) | ((w1 & 0x1FFFF) << 46);
+                        o = (w1 & P46) >> 17;
+                    }
+                    // ordered butterfly: higher lanes hold older tiles
+                    for (int m = 1; m < 64; m <<= 1) {
+                        const uint64_t oc = shfl_xor_u64(c, m), ow = shfl_xor_u64(w, m), oo = shfl_xor_u64(o, m);
+                        if (lane & m) w = win_combine(w, ow, oc, bo);
+                        else w = win_combine(ow, w, c, bo);
+                        c += oc;
+                        o = ovf_add(o, oo);
+                    }
+                    excl_w = win_combine(w, excl_w, excl_c, bo);
+                    excl_c += c;
+                    excl_o = ovf_add(excl_o, o);
+                    if (pm) break;
+                    pos -= 64;
+                }
+            }
+            const uint64_t incl_c = excl_c + ctile;
+            const uint64_t incl_w = win_combine(excl_w, own_w, ctile, bo);
+            const uint64_t incl_o = ovf_add(excl_o, otile);
+            if (lane == 0) {
+                st_rel(&a.lb.winP[2 * t], lb_word(2, epoch, incl_w));
+                st_rel(&a.lb.winP[2 * t + 1], lb_word(2, epoch, (incl_w >> 46) | (incl_o << 17)));
+                st_rel(&a.lb.status[t], lb_status(2, epoch, incl_c));
+                s_P = excl_c;
+                s_carry = excl_w;
+                if (t == T - 1) {
+                    if (a.stats) {
+                        a.stats[0] = incl_c;
+                        a.stats[1] = incl_o;
+                    }
+                    st32(&a.hdr->epoch, epoch + 1);
+                }
+            }
+        }
+        __syncthreads();
+        const uint64_t P = s_P, carry = s_carry;
+        const bool last = (t == T - 1);
+        const uint64_t ocap = a.cap;
+
+        // ---- 3. write
+        if (OK == K_STRIDED) {
+            uint32_t run = wpre;
+#pragma unroll
+            for (uint32_t j = 0; j < CS_J; j++) {
+                const uint64_t b = bal(j);
+                if ((b >> lane) & 1) {
+                    const uint64_t r = P + run + mbcnt(b);
+                    if (r < ocap) store_container(a.out, r, f[j].value(a.in.mask));
+                }
+                run += (uint32_t)__popcll(b);
+            }
+            if (last && tid == 0) {
+                uint64_t K = P + ctile;
+                if (K > ocap) K = ocap;
+                const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
+                for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
+            }
+        }
+        else if (OK == K_SLAB) {
+            const uint64_t eps = a.out.eps;
+            const uint64_t kf = P / eps;
+            uint64_t kl = (P + ctile) / eps;
+            if (last && ((P + ctile) % eps)) kl += 1;
+            const uint64_t wcap = (ocap + eps - 1) / eps;
+            if (kl > wcap) kl = wcap;
+            for (uint64_t k = kf + tid; k < kl; k += 256) {
+                uint64_t word = 0;
+                const uint64_t e0 = k * eps;
+                for (uint32_t s = 0; s < eps; s++) {
+                    const uint64_t e = e0 + s;
+                    if (e >= ocap) break;
+                    uint64_t val;
+                    if (e < P) val = (carry >> (63 - (P - e) * bo)) & om;
+                    else if (e < P + ctile) val = s_vals[e - P];
+                    else break;
+                    word |= val << (s * bo);
+                }
+                a.out.p[k] = word;
+            }
+        }
+        else {
+            const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
+            const uint64_t kf = B0 >> 6;
+            uint64_t kl = B1 >> 6;
+            if (last && (B1 & 63)) kl += 1;
+            const uint64_t wcap = (Bcap + 63) >> 6;
+            if (kl > wcap) kl = wcap;
+            for (uint64_t k = kf + tid; k < kl; k += 256) {
+                const uint64_t b0 = k * 64;
+                uint64_t word = 0;
+                if (b0 < B0) word = carry >> (b0 + 63 - B0);
+                uint64_t e0 = b0 / bo;
+                if (e0 < P) e0 = P;
+                uint64_t e1 = (b0 + 63) / bo + 1;
+                if (e1 > P + ctile) e1 = P + ctile;
+                for (uint64_t e = e0; e < e1; e++) {
+                    const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
+                    const uint64_t val = s_vals[e - P];
+                    word |= pos >= 0 ? (val << pos) : (val >> (-pos));
+                }
+                if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
+                a.out.p[k] = word;
+            }
+        }
+        __syncthreads(); // LDS reuse by the next tile
+#pragma unroll
+        for (uint32_t j = 0; j < CS_J; j++) f[j] = nf[j];
+        mk = nmk;
     }
 }
 
@@ -344,9 +359,20 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     a.lb = lb;
     hipStream_t s = (hipStream_t)stream;
     const bool use_pred = (mask == nullptr);
-#define CS_LAUNCH(IK, OK)                                                                              \
-    if (use_pred) hipLaunchKernelGGL((k_compress<IK, OK, true>), dim3((uint32_t)T), dim3(256), 0, s, a); \
-    else hipLaunchKernelGGL((k_compress<IK, OK, false>), dim3((uint32_t)T), dim3(256), 0, s, a);
+    const bool wide = ci.bits > 32;
+#define CS_GO(IK, OK, P, W)                                                                       \
+    {                                                                                             \
+        static uint32_t cap_blocks = 0;                                                           \
+        if (!cap_blocks) cap_blocks = resident_blocks(k_compress<IK, OK, P, W>, 256, 0);          \
+        const uint32_t g = (uint32_t)(T < cap_blocks ? T : cap_blocks);                           \
+        hipLaunchKernelGGL((k_compress<IK, OK, P, W>), dim3(g), dim3(256), 0, s, a);              \
+    }
+#define CS_LAUNCH(IK, OK)                                    \
+    if (use_pred) {                                          \
+        if (wide) CS_GO(IK, OK, true, true) else CS_GO(IK, OK, true, false)     \
+    } else {                                                 \
+        if (wide) CS_GO(IK, OK, false, true) else CS_GO(IK, OK, false, false)   \
+    }
 #define CS_OUT(IK)                                          \
     switch (co.kind) {                                      \
         case K_STRIDED: CS_LAUNCH(IK, K_STRIDED); break;    \
@@ -360,5 +386,6 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     }
 #undef CS_OUT
 #undef CS_LAUNCH
+#undef CS_GO
     return check_launch();
 }

@@ -11,67 +11,158 @@
 // with one coalesced store.
 //
 // Sum reference: kernel_hop_slabs / kernel_hop_nogaps (src/kernels/hop.cuh:7-76),
-// one global atomicAdd per thread. Here: per-block wave reductions, one
-// agent-scope atomic per block into the workspace, and the last block to
-// finish moves the total to the caller's pointer and re-zeroes the
-// accumulator (no memset, deterministic: u64 addition wraps and commutes).
+// one global atomicAdd per thread. Here each lane decodes whole packed words
+// (slab: eps fields with v_bfe; bitstream: the elements starting in its
+// word, straddlers completed from the next word), blocks write partial sums
+// to the workspace and a one-block kernel adds them: no contended atomics,
+// deterministic (u64 addition wraps and commutes).
 #include "am_device.hpp"
 #include "am_host.hpp"
 
 namespace am {
 
-constexpr uint32_t FT_J = 16;                 // mask words per wave
-constexpr uint32_t FT_TILE = 4 * FT_J * 64;   // elements per 256-thread block
+constexpr uint32_t FT_J = 16;                 // mask words per wave and tile
+constexpr uint32_t FT_TILE = 4 * FT_J * 64;   // elements per 256-thread tile
 
-template <int IK>
-__global__ __launch_bounds__(256) void k_filter(Col in, uint32_t pop, uint64_t pc, uint64_t* mask_out)
+// Persistent: block b filters tiles b, b+G, ... and loads tile t+G while it
+// evaluates tile t, so every wave keeps a tile's worth of loads in flight.
+template <int IK, bool WIDE>
+__global__ __launch_bounds__(256) void k_filter(Col in, uint32_t pop, uint64_t pc, uint64_t* mask_out, uint32_t T)
 {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t wbase = (uint64_t)blockIdx.x * FT_TILE + wave * (FT_J * 64);
-    if (wbase >= in.n) return;
-    TileReader<IK> rd(in, wbase);
-    uint64_t mine = 0;
+    const uint64_t n = in.n, words = (n + 63) >> 6;
+    uint32_t t = blockIdx.x;
+    if (t >= T) return;
+    Fetch<IK, WIDE> f[FT_J];
+    auto fetch_tile = [&](uint32_t tt, Fetch<IK, WIDE>* ff) {
+        const uint64_t wb = (uint64_t)tt * FT_TILE + wave * (FT_J * 64);
+        TileReader<IK> rd(in, wb < n ? wb : 0);
 #pragma unroll
-    for (uint32_t j = 0; j < FT_J; j++) {
-        const uint64_t e = wbase + j * 64 + lane;
-        bool f = false;
-        if (e < in.n) f = pred_eval(pop, pc, rd.get(j * 64 + lane));
-        uint64_t b = ballot(f);
-        if (lane == j) mine = b;
+        for (uint32_t j = 0; j < FT_J; j++) ff[j].load(rd, j * 64 + lane, wb + j * 64 + lane < n);
+    };
+    fetch_tile(t, f);
+    for (; t < T; t += gridDim.x) {
+        Fetch<IK, WIDE> nf[FT_J];
+        if (t + gridDim.x < T) fetch_tile(t + gridDim.x, nf);
+        const uint64_t wbase = (uint64_t)t * FT_TILE + wave * (FT_J * 64);
+        uint64_t mine = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < FT_J; j++) {
+            const bool fl = (wbase + j * 64 + lane < n) && pred_eval(pop, pc, f[j].value(in.mask));
+            const uint64_t b = ballot(fl);
+            if (lane == j) mine = b;
+        }
+        const uint64_t w = (wbase >> 6) + lane;
+        if (lane < FT_J && w < words) mask_out[w] = mine;
+#pragma unroll
+        for (uint32_t j = 0; j < FT_J; j++) f[j] = nf[j];
     }
-    const uint64_t words = (in.n + 63) >> 6;
-    const uint64_t w = (wbase >> 6) + lane;
-    if (lane < FT_J && w < words) mask_out[w] = mine;
 }
 
-template <int IK>
-__global__ __launch_bounds__(256) void k_sum(Col in, WsHeader* hdr, uint64_t* out)
+// ---- sum: lane-per-word decode, per-block partials, then one final block.
+constexpr uint32_t SUM_UNROLL = 4;
+
+// sum of the elements of slab word q (elements q*eps .. q*eps+eps-1, < n)
+__device__ inline uint64_t slab_word_sum(uint64_t w, uint64_t q, const Col& c)
+{
+    uint64_t first = q * c.eps;
+    uint32_t cnt = c.eps;
+    if (first + cnt > c.n) cnt = (uint32_t)(c.n - first);
+    uint64_t s = 0;
+    if (c.bits <= 32) {
+        const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+        for (uint32_t k = 0; k < cnt; k++) {
+            const uint32_t off = k * c.bits;
+            uint32_t v;
+            if (off + c.bits <= 32) v = __builtin_amdgcn_ubfe(lo, off, c.bits);
+            else if (off >= 32) v = __builtin_amdgcn_ubfe(hi, off - 32, c.bits);
+            else v = (uint32_t)((w >> off) & c.mask);
+            s += v;
+        }
+    }
+    else {
+        for (uint32_t k = 0; k < cnt; k++) s += (w >> (k * c.bits)) & c.mask;
+    }
+    return s;
+}
+
+template <int IK, bool WIDE>
+__global__ __launch_bounds__(256) void k_sum_partial(Col in, uint64_t* partial)
 {
     __shared__ uint64_t s_part[4];
     uint64_t acc = 0;
-    for (uint64_t t0 = (uint64_t)blockIdx.x * FT_TILE; t0 < in.n; t0 += (uint64_t)gridDim.x * FT_TILE) {
-        TileReader<IK> rd(in, t0);
-        const uint64_t rem = in.n - t0;
+    const uint64_t gstride = (uint64_t)gridDim.x * 256 * SUM_UNROLL;
+    if (IK == K_SLAB) {
+        const uint64_t words = (in.n + in.eps - 1) / in.eps;
+        for (uint64_t q0 = (uint64_t)blockIdx.x * 256 * SUM_UNROLL + threadIdx.x; q0 < words; q0 += gstride) {
+            uint64_t wv[SUM_UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+                const uint64_t q = q0 + u * 256;
+                wv[u] = q < words ? in.p[q] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+                const uint64_t q = q0 + u * 256;
+                if (q < words) acc += slab_word_sum(wv[u], q, in);
+            }
+        }
+    }
+    else if (IK == K_BITSTREAM && !WIDE) {
+        // lane owns the elements that START in its word; a straddler takes
+        // its top bits from the next word
+        const uint64_t words = (in.n * in.bits + 63) >> 6;
+        for (uint64_t q0 = (uint64_t)blockIdx.x * 256 * SUM_UNROLL + threadIdx.x; q0 < words; q0 += gstride) {
+            uint64_t wv[SUM_UNROLL], nv[SUM_UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+                const uint64_t q = q0 + u * 256;
+                wv[u] = q < words ? in.p[q] : 0;
+                nv[u] = q + 1 < words ? in.p[q + 1] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+                const uint64_t q = q0 + u * 256;
+                if (q >= words) continue;
+                const uint64_t b0 = q * 64;
+                uint64_t e = (b0 + in.bits - 1) / in.bits; // first element starting in this word
+                uint64_t bit = e * in.bits;
+                for (; bit < b0 + 64 && e < in.n; bit += in.bits, e++) {
+                    const uint32_t off = (uint32_t)(bit - b0);
+                    uint64_t v = wv[u] >> off;
+                    if (off + in.bits > 64) v |= nv[u] << (64 - off);
+                    acc += v & in.mask;
+                }
+            }
+        }
+    }
+    else {
+        // strided containers (and wide bitstreams): element per lane
+        for (uint64_t t0 = (uint64_t)blockIdx.x * FT_TILE; t0 < in.n; t0 += (uint64_t)gridDim.x * FT_TILE) {
+            TileReader<IK> rd(in, t0);
+            const uint64_t rem = in.n - t0;
 #pragma unroll 4
-        for (uint32_t r = 0; r < FT_TILE / 256; r++) {
-            uint32_t i = r * 256 + threadIdx.x;
-            if (i < rem) acc += rd.get(i);
+            for (uint32_t r = 0; r < FT_TILE / 256; r++) {
+                const uint32_t i = r * 256 + threadIdx.x;
+                if (i < rem) acc += rd.get(i);
+            }
         }
     }
     acc = wave_sum_u64(acc);
     if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t tot = s_part[0] + s_part[1] + s_part[2] + s_part[3];
-        (void)add64(&hdr->accum, tot);
-        drain();
-        uint32_t d = add32(&hdr->done, 1);
-        if (d == gridDim.x - 1) {
-            uint64_t total = __hip_atomic_exchange(&hdr->accum, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            out[0] = total;
-            st32(&hdr->done, 0);
-        }
-    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+}
+
+__global__ __launch_bounds__(256) void k_sum_final(const uint64_t* partial, uint32_t nparts, uint64_t* out)
+{
+    __shared__ uint64_t s_part[4];
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += 256) acc += partial[i];
+    acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 }
 
 } // namespace am
@@ -86,13 +177,23 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
     if (pred.op > AM_PRED_NE) return AM_E_INVALID;
     if (!mask_out && c.n) return AM_E_INVALID;
     if (c.n == 0) return AM_OK;
-    dim3 grid((uint32_t)((c.n + FT_TILE - 1) / FT_TILE));
+    const uint64_t T = (c.n + FT_TILE - 1) / FT_TILE;
+    if (T >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
     hipStream_t s = (hipStream_t)stream;
-    switch (c.kind) {
-        case K_STRIDED: hipLaunchKernelGGL(k_filter<K_STRIDED>, grid, dim3(256), 0, s, c, pred.op, pred.c, mask_out); break;
-        case K_SLAB: hipLaunchKernelGGL(k_filter<K_SLAB>, grid, dim3(256), 0, s, c, pred.op, pred.c, mask_out); break;
-        default: hipLaunchKernelGGL(k_filter<K_BITSTREAM>, grid, dim3(256), 0, s, c, pred.op, pred.c, mask_out); break;
+#define FT_GO(IK, W)                                                                                        \
+    {                                                                                                       \
+        static uint32_t capb = 0;                                                                           \
+        if (!capb) capb = resident_blocks(k_filter<IK, W>, 256, 0);                                         \
+        const uint32_t g = (uint32_t)(T < capb ? T : capb);                                                 \
+        hipLaunchKernelGGL((k_filter<IK, W>), dim3(g), dim3(256), 0, s, c, pred.op, pred.c, mask_out, (uint32_t)T); \
     }
+    const bool wide = c.bits > 32;
+    switch (c.kind) {
+        case K_STRIDED: if (wide) FT_GO(K_STRIDED, true) else FT_GO(K_STRIDED, false) break;
+        case K_SLAB: if (wide) FT_GO(K_SLAB, true) else FT_GO(K_SLAB, false) break;
+        default: if (wide) FT_GO(K_BITSTREAM, true) else FT_GO(K_BITSTREAM, false) break;
+    }
+#undef FT_GO
     return check_launch();
 }
 
@@ -103,14 +204,23 @@ extern "C" int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t
     if (st) return st;
     if (!sum_dev) return AM_E_INVALID;
     if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
-    uint64_t tiles = (c.n + FT_TILE - 1) / FT_TILE;
-    uint32_t blocks = (uint32_t)(tiles < 2048 ? (tiles ? tiles : 1) : 2048);
+    uint64_t units = c.kind == K_SLAB ? (c.n + c.eps - 1) / c.eps
+                   : (c.kind == K_BITSTREAM && c.bits <= 32 ? (c.n * c.bits + 63) / 64 : (c.n + 3) / 4);
+    uint64_t blocks = (units + 256 * SUM_UNROLL - 1) / (256 * SUM_UNROLL);
+    if (blocks < 1) blocks = 1;
+    if (blocks > 2048) blocks = 2048;
+    uint64_t* partial = ws_small(ws);
     hipStream_t s = (hipStream_t)stream;
-    WsHeader* h = (WsHeader*)ws;
+    const bool wide = c.bits > 32;
+    dim3 g((uint32_t)blocks), b(256);
     switch (c.kind) {
-        case K_STRIDED: hipLaunchKernelGGL(k_sum<K_STRIDED>, dim3(blocks), dim3(256), 0, s, c, h, sum_dev); break;
-        case K_SLAB: hipLaunchKernelGGL(k_sum<K_SLAB>, dim3(blocks), dim3(256), 0, s, c, h, sum_dev); break;
-        default: hipLaunchKernelGGL(k_sum<K_BITSTREAM>, dim3(blocks), dim3(256), 0, s, c, h, sum_dev); break;
+        case K_STRIDED: hipLaunchKernelGGL((k_sum_partial<K_STRIDED, false>), g, b, 0, s, c, partial); break;
+        case K_SLAB: hipLaunchKernelGGL((k_sum_partial<K_SLAB, false>), g, b, 0, s, c, partial); break;
+        default:
+            if (wide) hipLaunchKernelGGL((k_sum_partial<K_BITSTREAM, true>), g, b, 0, s, c, partial);
+            else hipLaunchKernelGGL((k_sum_partial<K_BITSTREAM, false>), g, b, 0, s, c, partial);
+            break;
     }
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partial, (uint32_t)blocks, sum_dev);
     return check_launch();
 }

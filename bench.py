@@ -1,0 +1,287 @@
+"""Benchmark: achieved HBM GB/s of the packed-column operators on one MI355X.
+
+Headline step (the JSON line's `value`): one fused filter + compressstore over
+a 1 GB-equivalent column -- N = 2^28 uint32 values uniform in [0, 2^11)
+(PCG32 seed 42, generated on the GPU), slab-packed at 11 bits, predicate
+v < 1024 (~50 % selectivity), output uncompressed uint32 + LSB-first bitmask.
+value = algorithmic bytes / time, bytes = packed input + 4*K + N/8.
+
+The rest of the operator sweep (widths {4,8,11,17,24,32}, output formats,
+selectivities, mask shapes, binary add/and, u64 sum, group-by) is measured in
+the same run and reported under "ops" (median of repeated launches, HIP
+events on the launch stream).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0
+N_FULL = 1 << 28
+WIDTHS = (4, 8, 11, 17, 24, 32)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--quick", action="store_true", help="headline only, no sweep")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--reps", type=int, default=10, help="launches per sweep point")
+    p.add_argument("--profile-copy", action="store_true",
+                   help="also run a plain 1 GiB device copy (reference point for HBM counters)")
+    return p.parse_args()
+
+
+def gbytes(b, ms):
+    return b / (ms * 1e-3) / 1e9
+
+
+class Timer:
+    """Times launches on torch's current stream (the stream the ops run on)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def median_ms(self, fn, reps, warm=2):
+        torch = self.torch
+        for _ in range(warm):
+            fn()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in evs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        return statistics.median(a.elapsed_time(b) for a, b in evs)
+
+
+def read_traffic(name):
+    """FETCH/WRITE KiB per dispatch from a committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line:
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    if "fetch_kib_per_dispatch" not in vals or "write_kib_per_dispatch" not in vals:
+        return None
+    scale = vals.get("copy_scale", 1.0)
+    return (vals["fetch_kib_per_dispatch"] + vals["write_kib_per_dispatch"]) * 1024.0 * scale
+
+
+def cpu_baseline(am, torch, col, pred_c):
+    """The oracle's restatement of the reference's host loop, timed on the host
+    on a bounded sample: the first 2^24 values of the same column."""
+    import oracle
+    nsamp = 1 << 24
+    bits = col.bits
+    words = col.buf[:am.packed_words(am.SLAB, bits, nsamp)].cpu().numpy().view(np.uint64).copy()
+    k = None
+    res = {}
+    for threads in (1, oracle.num_threads()):
+        times = []
+        for rep in range(4):
+            t0 = time.perf_counter()
+            k, out, mask = oracle.cpu_filter_compress_u32(bits, words, nsamp, am.LT, pred_c, threads=threads)
+            times.append(time.perf_counter() - t0)
+        t = statistics.median(times[1:])
+        b = am.packed_words(am.SLAB, bits, nsamp) * 8 + 4 * k + (nsamp + 63) // 64 * 8
+        res[threads] = (b / t / 1e9, t)
+    st = res[1]
+    mt_threads = max(res)
+    return {"value": round(st[0], 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"first 2^24 of the 2^28 w=11 slab column (filter v<1024 + compressstore to u32 + mask), "
+                      f"single-thread faithful per-element loop, median of 3 after 1 warm-up ({st[1]*1e3:.1f} ms)",
+            "parallel": {"value": round(res[mt_threads][0], 3), "cores": mt_threads,
+                         "ms": round(res[mt_threads][1] * 1e3, 2)}}
+
+
+def sweep(am, torch, timer, reps):
+    """Per-op achieved GB/s at N = 2^28 (algorithmic bytes)."""
+    ops = []
+    n = N_FULL
+    ws = am.Workspace(n)
+    stats = am.device_words(2)
+    mask_words = (n + 63) // 64
+    for w in WIDTHS:
+        print(f"sweep w={w}", file=sys.stderr, flush=True)
+        a = am.gen_uniform(42, n, w, am.SLAB)
+        b = am.gen_uniform(43, n, w, am.SLAB)
+        in_b = a.words * 8
+        torch.cuda.synchronize()
+
+        # filter -> mask
+        mo = am.device_words(mask_words)
+        ms = timer.median_ms(lambda: am.filter(a, am.LT, 1 << (w - 1), mask_out=mo), reps)
+        ops.append({"op": "filter", "w": w, "in": "slab", "out": "mask", "ms": ms,
+                    "GBs": gbytes(in_b + mask_words * 8, ms)})
+
+        # fused filter + compressstore at 1/50/99 % selectivity, three output formats
+        for frac in (0.01, 0.5, 0.99):
+            c = max(1, int(round(frac * (1 << w))))
+            pw = am.pred_width(am.LT, c, w)
+            for fo, ob, label in ((am.U32, w, "u32"), (am.SLAB, w, "slab_same_w"), (am.SLAB, pw, "slab_min_w")):
+                out = am.Column(n, ob, fo)
+                am.compressstore(a, ob, fo, pred=(am.LT, c), ws=ws, out=out, stats=stats)
+                torch.cuda.synchronize()
+                k = int(am.u64(stats)[0])
+                ms = timer.median_ms(lambda: am.compressstore(a, ob, fo, pred=(am.LT, c), ws=ws, out=out, stats=stats),
+                                     reps)
+                byt = in_b + am.packed_words(fo, ob, k) * 8
+                ops.append({"op": "filter+compressstore", "w": w, "sel": frac, "in": "slab", "out": label, "ob": ob,
+                            "ms": ms, "GBs": gbytes(byt, ms)})
+                del out
+
+        # mask-driven compressstore (reference semantics: slab -> slab same w), uniform and clustered masks
+        for frac in (0.01, 0.5, 0.99):
+            for shape in ("uniform", "clustered"):
+                m = (am.gen_mask_bernoulli(7, n, frac) if shape == "uniform" else am.gen_mask_runs(7, n, frac, 256.0))
+                out = am.Column(n, w, am.SLAB)
+                am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats)
+                torch.cuda.synchronize()
+                k = int(am.u64(stats)[0])
+                ms = timer.median_ms(lambda: am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats), reps)
+                byt = in_b + mask_words * 8 + am.packed_words(am.SLAB, w, k) * 8
+                ops.append({"op": "compressstore(mask)", "w": w, "sel": frac, "mask": shape, "in": "slab",
+                            "out": "slab_same_w", "ms": ms, "GBs": gbytes(byt, ms)})
+                del out, m
+
+        # binary add (w+1) and and (w)
+        for op, name in ((am.ADD, "add"), (am.AND, "and")):
+            ob = am.outbits(op, w)
+            out = am.Column(n, ob, am.SLAB)
+            ms = timer.median_ms(lambda: am.binary(op, a, b, out=out), reps)
+            ops.append({"op": name, "w": w, "in": "slab", "out": f"slab w={ob}", "ms": ms,
+                        "GBs": gbytes(2 * in_b + out.words * 8, ms)})
+            del out
+
+        # u64 sum
+        s = am.device_words(1)
+        ms = timer.median_ms(lambda: am.sum(a, ws=ws, out=s), reps)
+        ops.append({"op": "sum", "w": w, "in": "slab", "ms": ms, "GBs": gbytes(in_b + 8, ms)})
+
+        # bitstream variants at w = 11
+        if w == 11:
+            ab = am.convert(a, w, am.BITSTREAM)
+            bb = am.convert(b, w, am.BITSTREAM)
+            ib = ab.words * 8
+            ms = timer.median_ms(lambda: am.filter(ab, am.LT, 1 << (w - 1), mask_out=mo), reps)
+            ops.append({"op": "filter", "w": w, "in": "bitstream", "out": "mask", "ms": ms,
+                        "GBs": gbytes(ib + mask_words * 8, ms)})
+            out = am.Column(n, w + 1, am.BITSTREAM)
+            ms = timer.median_ms(lambda: am.binary(am.ADD, ab, bb, out=out), reps)
+            ops.append({"op": "add", "w": w, "in": "bitstream", "out": "bitstream w=12", "ms": ms,
+                        "GBs": gbytes(2 * ib + out.words * 8, ms)})
+            ms = timer.median_ms(lambda: am.sum(ab, ws=ws, out=s), reps)
+            ops.append({"op": "sum", "w": w, "in": "bitstream", "ms": ms, "GBs": gbytes(ib + 8, ms)})
+            del ab, bb, out
+        del a, b
+        torch.cuda.empty_cache()
+    for o in ops:
+        o["ms"] = round(o["ms"], 4)
+        o["GBs"] = round(o["GBs"], 1)
+        o["pct_peak"] = round(100.0 * o["GBs"] / PEAK_GBS, 1)
+    return ops
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the operators are single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import amethyst_amd as am
+    am.lib()
+    torch.cuda.init()
+    timer = Timer(torch)
+    if args.profile_copy:
+        src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    # ---- headline workload
+    n, w, c = N_FULL, 11, 1024
+    col = am.gen_uniform(42, n, w, am.SLAB)
+    out = am.Column(n, 32, am.U32)
+    ws = am.Workspace(n)
+    stats = am.device_words(2)
+    mask_out = am.device_words((n + 63) // 64)
+
+    def step():
+        am.compressstore(col, 32, am.U32, pred=(am.LT, c), ws=ws, out=out, stats=stats, mask_out=mask_out)
+
+    step()
+    torch.cuda.synchronize()
+    k = int(am.u64(stats)[0])
+    byt = col.words * 8 + 4 * k + ((n + 63) // 64) * 8
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # per-launch kernel time with HIP events on the launch stream
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a_, b_ in evs:
+        a_.record()
+        step()
+        b_.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a_.elapsed_time(b_) for a_, b_ in evs]
+    ms_step = wall * 1e3 / args.steps
+    avg_kernel = sum(kms) / len(kms)
+    value = gbytes(byt, ms_step)
+    achieved = gbytes(byt, avg_kernel)
+    traffic = read_traffic("k_compress_w11_u32_traffic.txt")
+
+    ops = [] if args.quick else sweep(am, torch, timer, args.reps)
+    print("cpu baseline", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(am, torch, col, c)
+
+    line = {
+        "metric": "HBM GB/s (% of peak) per op; batched+learned plan vs plain plan speedup",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (PCG32 seed 42, generated on the GPU)",
+        "config": {"workload": "filter(v<1024)+compressstore, N=2^28 uniform 11-bit values, slab in, "
+                               "u32 out + bitmask, one MI355X", "n": n, "bits": w, "selected": k,
+                   "bytes_per_step": byt},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_compress<K_SLAB,K_STRIDED,true>", "avg_kernel_ms": round(avg_kernel, 5)},
+        "cpu_baseline": cpu,
+        "ops": ops,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -150,8 +150,26 @@ int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream);
  * computed in parallel with PCG jump-ahead: out[i] = draw (offset + i). */
 int am_prng_fill_u32(uint64_t seed, uint64_t offset, uint32_t* out, uint64_t n, void* stream);
 
-/* Uniform w-bit values v_i = draw(i) & mask(bits) packed into `out`. */
+/* Uniform w-bit values v_i = draw(i) & mask(bits) packed into `out`
+ * (bits > 32: v_i = (draw(2i) << 32 | draw(2i+1)) & mask). The reference
+ * fills its inputs the same way on the host (src/main.cu:239-251). */
 int am_gen_uniform(uint64_t seed, am_column* out, void* stream);
+
+/* Inverse-CDF draws: v_i = first b with draw(i) < thresholds_dev[b]
+ * (device array of nbins ascending uint32 thresholds, last = 2^32-1);
+ * used for Zipf-distributed group-by keys. No reference counterpart. */
+int am_gen_from_cdf(uint64_t seed, const uint32_t* thresholds_dev, uint32_t nbins, am_column* out, void* stream);
+
+/* Bernoulli(selectivity) mask: bit i = draw(i) < round(s * 2^32). LSB-first
+ * counterpart of generate_mask_uniform (src/main.cu:33-46), which writes
+ * MSB-first bytes from a serial stream. */
+int am_gen_mask_bernoulli(uint64_t seed, double selectivity, uint64_t* mask, uint64_t n, void* stream);
+
+/* Clustered mask: alternating runs, 1-runs Geometric(mean mean_run),
+ * 0-runs Geometric(mean mean_run*(1-s)/s), from a two-state Markov chain
+ * restarted from its stationary distribution every 4096 elements.
+ * Counterpart of create_bitmask's clustered masks (src/main.cu:59-108). */
+int am_gen_mask_runs(uint64_t seed, double selectivity, double mean_run, uint64_t* mask, uint64_t n, void* stream);
 
 /* ---------------------------------------------------------------- operators */
 
@@ -213,6 +231,39 @@ int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t ws_bytes, 
 int am_groupby(const am_column* keys, const am_column* values, const am_predicate* value_pred,
                uint32_t mode, uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts,
                uint64_t* result_dev, void* ws, uint64_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------- packed matmul */
+
+/* One product C = A * B of unsigned `bits`-bit matrices (bits <= 7), A M x K
+ * and B K x N, both row-major with each row padded to whole slabs -- the
+ * layout of get_matrix_element (src/common_code.cuh:124-129). C_packed
+ * (may be NULL) receives C as a row-padded slab matrix at out_bits;
+ * C_i32 (may be NULL) receives it as row-major int32. The reference has
+ * only the layout helpers (src/common_code.cuh:65-86,118-134); its matrix
+ * kernels were never released (README.md:20). */
+typedef struct {
+    const uint64_t* A;
+    const uint64_t* B;
+    uint64_t* C_packed;
+    int32_t* C_i32;
+    uint32_t M, N, K;
+    uint32_t out_bits; /* 0: am_matmul_out_bits(bits, K) */
+} am_matmul_desc;
+
+/* Width of the largest possible entry, K * (2^bits - 1)^2. */
+uint32_t am_matmul_out_bits(uint32_t bits, uint32_t K);
+
+/* Workspace for a batch of `count` products with the given shapes. */
+uint64_t am_matmul_workspace_bytes(const uint32_t* M, const uint32_t* N, const uint32_t* K, uint32_t count);
+
+/* Exact int32 products on the int8 matrix cores (K * (2^bits-1)^2 < 2^31). */
+int am_matmul_packed(uint32_t bits, const uint64_t* A, const uint64_t* B, uint32_t M, uint32_t N, uint32_t K,
+                     uint32_t out_bits, uint64_t* C_packed, int32_t* C_i32, void* ws, uint64_t ws_bytes, void* stream);
+
+/* A batch of uneven products (descs in host memory) with one launch per
+ * phase for the whole batch. */
+int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws, uint64_t ws_bytes,
+                             void* stream);
 
 #ifdef __cplusplus
 }

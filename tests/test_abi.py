@@ -1,0 +1,96 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/*.h declares, and its pure host helpers follow the reference's
+width rules. No kernel is launched here."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(am_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def L():
+    import amethyst_amd
+    return amethyst_amd.lib()
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("am_filter", "am_compressstore", "am_binary", "am_sum", "am_groupby", "am_convert"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(L):
+    path = os.path.join(ROOT, "amethyst_amd", "libamethyst.so")
+    lib = ctypes.CDLL(path)
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_is_built_for_gfx950():
+    """The device code object bundled into the library targets gfx950."""
+    path = os.path.join(ROOT, "amethyst_amd", "libamethyst.so")
+    blob = open(path, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_packed_words(L):
+    import amethyst_amd as am
+    n = 1000
+    assert am.packed_words(am.BASELINE, 11, n) == n
+    assert am.packed_words(am.SLAB, 11, n) == -(-n // 5)
+    assert am.packed_words(am.BITSTREAM, 11, n) == -(-(n * 11) // 64)
+    assert am.packed_words(am.U32, 11, n) == n // 2
+    assert am.packed_words(am.TIGHT, 12, n) == -(-(n * 2) // 8)
+    assert am.packed_words(am.U32, 33, n) == 0  # invalid combination
+    assert am.packed_words(am.SLAB, 0, n) == 0
+
+
+def test_outbits_and_pred_width(L):
+    import amethyst_amd as am
+    # op_add_outbits / op_mul_outbits / op_mod_outbits (src/config.cuh:44-55)
+    assert [am.outbits(am.ADD, w) for w in (1, 11, 63, 64)] == [2, 12, 64, 64]
+    assert [am.outbits(am.MUL, w) for w in (1, 11, 32, 40)] == [2, 22, 64, 64]
+    assert [am.outbits(am.MOD, w) for w in (1, 11, 64)] == [1, 11, 64]
+    assert am.pred_width(am.LT, 1024, 11) == 10
+    assert am.pred_width(am.LE, 1024, 11) == 11
+    assert am.pred_width(am.GE, 1024, 11) == 11
+    assert am.pred_width(am.LT, 1, 11) == 1
+    assert am.pred_width(am.EQ, 5, 11) == 3
+    assert L.am_gen_bitmask(64) == 2**64 - 1
+    assert L.am_gen_bitmask(11) == 2047
+
+
+def test_status_strings_and_errors(L):
+    import amethyst_amd as am
+    assert L.am_status_string(0) == b"ok"
+    assert L.am_status_string(2) == b"unsupported configuration"
+    # invalid arguments are rejected on the host before any launch
+    bad = am.am_column(None, 10, 11, am.SLAB)
+    ok = am.am_column(None, 0, 11, am.SLAB)
+    assert L.am_convert(ctypes.byref(bad), ctypes.byref(ok), None) == 1
+    col65 = am.am_column(None, 0, 65, am.SLAB)
+    assert L.am_convert(ctypes.byref(col65), ctypes.byref(ok), None) == 1
+    u33 = am.am_column(None, 0, 33, am.U32)
+    assert L.am_filter(ctypes.byref(u33), am.am_predicate(0, 0, 0), None, None) == 1
+    keys13 = am.am_column(ctypes.c_void_p(8), 1, 13, am.SLAB)
+    vals = am.am_column(ctypes.c_void_p(8), 1, 20, am.SLAB)
+    dummy = ctypes.c_void_p(8)
+    # keys wider than 12 bits: unsupported (the reference caps shared-memory group-by at 12 bits)
+    assert L.am_groupby(ctypes.byref(keys13), ctypes.byref(vals), None, 0, dummy, dummy, dummy, dummy, dummy,
+                        1 << 30, None) == 2
+    # missing workspace
+    assert L.am_sum(ctypes.byref(vals), dummy, None, 0, None) == 4

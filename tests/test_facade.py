@@ -1,0 +1,22 @@
+"""The C++ facade (include/amethyst.hpp) builds with g++ and its iterators
+match the reference's bit_iterator semantics (golden vectors from the
+reference's own src/bit_iterator.cuh). Host only."""
+import json
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_facade_iterators(tmp_path):
+    exe = tmp_path / "facade"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "cpp", "facade_iterators.cpp"), "-o", str(exe),
+                           "-L", os.path.join(ROOT, "amethyst_amd"), "-lamethyst",
+                           "-Wl,-rpath," + os.path.join(ROOT, "amethyst_amd") + ":/opt/rocm/lib"])
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))["bit_iterator"]
+    out = subprocess.run([str(exe)] + ["%02x" % b for b in g["bytes"]], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    bits = out.stdout.splitlines()[0]
+    assert [int(c) for c in bits] == g["index"]
+    assert out.stdout.strip().endswith("OK")

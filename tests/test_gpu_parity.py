@@ -205,3 +205,93 @@ def test_sum(am, oracle_mod):
                 got = int(am.u64(am.sum(col, ws=ws))[0])
                 exp = oracle_mod.sum(fmt, bits, oracle_mod.pack(fmt, bits, vals), n)
                 assert got == exp, (bits, n, fmt)
+
+
+def _width(v):
+    return max(1, int(v).bit_length())
+
+
+def test_groupby(am, oracle_mod):
+    rng = np.random.default_rng(9)
+    ws = am.Workspace(0)
+    for kbits, vbits in ((10, 20), (12, 32), (1, 5), (4, 1)):
+        for n in (1, 4097, 100003, 2000003):
+            g = 1 << kbits
+            # Zipf-like skew: rank r with weight 1/(r+1)^1.1
+            w = 1.0 / np.arange(1, g + 1) ** 1.1
+            keys = rng.choice(g, size=n, p=w / w.sum()).astype(np.uint64)
+            vals = rand_vals(rng, n, vbits)
+            for kf, vf in ((1, 1), (2, 2), (4, 3)):
+                kc = upload(am, oracle_mod, keys, kbits, kf)
+                vc = upload(am, oracle_mod, vals, vbits, vf)
+                for mode in (0, 1):
+                    for pred in (None, (2, (1 << (vbits - 1)) - 1)):
+                        say("groupby", kbits, vbits, n, kf, vf, mode, pred)
+                        ok, osum, ocnt, res = am.groupby(kc, vc, mode=mode, value_pred=pred, ws=ws)
+                        r = am.u64(res)
+                        sums, counts, rows = oracle_mod.groupby(kf, kbits, oracle_mod.pack(kf, kbits, keys), vf, vbits,
+                                                                oracle_mod.pack(vf, vbits, vals), n, pred)
+                        present = np.nonzero(counts)[0]
+                        ng = len(present)
+                        exp_s = sums[present]
+                        if mode == 1:
+                            exp_s = exp_s & np.uint64((1 << vbits) - 1)
+                        sb = vbits if mode == 1 else _width(exp_s.max() if ng else 0)
+                        cb = _width(counts.max())
+                        assert list(map(int, r)) == [ng, sb, cb, rows], (list(map(int, r)), ng, sb, cb, rows)
+                        assert np.array_equal(oracle_mod.unpack(1, kbits, am.u64(ok), ng), present.astype(np.uint64))
+                        assert np.array_equal(oracle_mod.unpack(1, sb, am.u64(osum), ng), exp_s)
+                        assert np.array_equal(oracle_mod.unpack(1, cb, am.u64(ocnt), ng), counts[present])
+                        # dead bits of the last packed word are zero
+                        for buf, bits in ((ok, kbits), (osum, sb), (ocnt, cb)):
+                            wds = oracle_mod.packed_words(1, bits, ng)
+                            assert np.array_equal(am.u64(buf)[:wds], oracle_mod.pack(1, bits, oracle_mod.unpack(
+                                1, bits, am.u64(buf), ng)))
+
+
+def _pack_matrix(oracle_mod, X, bits):
+    return np.concatenate([oracle_mod.pack(1, bits, X[r]) for r in range(X.shape[0])])
+
+
+def test_matmul_packed(am, oracle_mod):
+    import torch
+    rng = np.random.default_rng(10)
+    shapes = [(1, 1, 1), (16, 16, 64), (17, 33, 65), (128, 128, 64), (130, 200, 300), (257, 129, 1000)]
+    for bits in (7, 3, 1):
+        probs, exps = [], []
+        for (M, N, K) in shapes:
+            A = rng.integers(0, 1 << bits, (M, K)).astype(np.uint64)
+            B = rng.integers(0, 1 << bits, (K, N)).astype(np.uint64)
+            pa, pb = _pack_matrix(oracle_mod, A, bits), _pack_matrix(oracle_mod, B, bits)
+            exps.append((oracle_mod.matmul_packed(bits, pa, pb, M, N, K), A, B))
+            probs.append((torch.from_numpy(pa.view(np.int64)).cuda(), torch.from_numpy(pb.view(np.int64)).cuda(), M, N, K))
+        say("matmul batch", bits, shapes)
+        outs = am.matmul_packed_batched(bits, probs, want_i32=True)
+        torch.cuda.synchronize()
+        for (cp, ci, ob), (C, A, B), (_, _, M, N, K) in zip(outs, exps, probs):
+            assert np.array_equal(C, A.astype(np.int64) @ B.astype(np.int64))
+            got = ci.cpu().numpy().reshape(M, N).astype(np.int64)
+            assert np.array_equal(got, C), (bits, M, N, K)
+            assert ob == _width(K * ((1 << bits) - 1) ** 2)
+            exp_words = _pack_matrix(oracle_mod, C.astype(np.uint64), ob)
+            assert np.array_equal(am.u64(cp)[:len(exp_words)], exp_words), (bits, M, N, K, ob)
+
+
+def test_matmul_uneven_batch_packed_only(am, oracle_mod):
+    """A batch of uneven shapes with packed output only (row packer path)."""
+    import torch
+    rng = np.random.default_rng(11)
+    probs, exps = [], []
+    for _ in range(12):
+        M, N, K = (int(x) for x in rng.integers(64, 400, 3))
+        A = rng.integers(0, 128, (M, K)).astype(np.uint64)
+        B = rng.integers(0, 128, (K, N)).astype(np.uint64)
+        pa, pb = _pack_matrix(oracle_mod, A, 7), _pack_matrix(oracle_mod, B, 7)
+        exps.append(A.astype(np.int64) @ B.astype(np.int64))
+        probs.append((torch.from_numpy(pa.view(np.int64)).cuda(), torch.from_numpy(pb.view(np.int64)).cuda(), M, N, K))
+    say("matmul uneven batch")
+    outs = am.matmul_packed_batched(7, probs)
+    torch.cuda.synchronize()
+    for (cp, _, ob), C in zip(outs, exps):
+        exp_words = _pack_matrix(oracle_mod, C.astype(np.uint64), ob)
+        assert np.array_equal(am.u64(cp)[:len(exp_words)], exp_words)
