@@ -54,6 +54,11 @@ class am_predicate(ctypes.Structure):
     _fields_ = [("op", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("c", ctypes.c_uint64)]
 
 
+class am_pipeline_col(ctypes.Structure):
+    _fields_ = [("a", am_column), ("b", am_column), ("s", am_column), ("pred", am_predicate),
+                ("stats", ctypes.c_void_p)]
+
+
 def build(verbose=False):
     """Compile the HIP library in-tree for gfx950 (hipcc via make)."""
     import subprocess
@@ -87,12 +92,19 @@ def lib():
             "am_gen_from_cdf": ([u64, vp, u32, colp, vp], i32),
             "am_gen_mask_bernoulli": ([u64, ctypes.c_double, vp, u64, vp], i32),
             "am_gen_mask_runs": ([u64, ctypes.c_double, ctypes.c_double, vp, u64, vp], i32),
+            "am_gen_sorted_noise": ([u64, u32, colp, vp], i32),
             "am_convert": ([colp, colp, vp], i32),
             "am_filter": ([colp, am_predicate, vp, vp], i32),
             "am_compressstore": ([colp, vp, predp, colp, vp, vp, vp, u64, vp], i32),
             "am_binary": ([u32, colp, colp, colp, vp], i32),
             "am_sum": ([colp, vp, vp, u64, vp], i32),
             "am_groupby": ([colp, colp, predp, u32, vp, vp, vp, vp, vp, u64, vp], i32),
+            "am_pipeline_workspace_bytes": ([vp, u32], u64),
+            "am_pipeline_prepare": ([vp, u32, u32, vp, u64, vp], i32),
+            "am_pipeline_run": ([u32, u32, u32, u32, vp, u64, vp], i32),
+            "am_plan_scratch_bytes": ([u32], u64),
+            "am_plan_features": ([vp, u32, vp, vp, vp], i32),
+            "am_plan_infer": ([vp, u32, vp, u32, vp, vp, ctypes.c_float, vp, vp, vp, vp], i32),
             "am_matmul_out_bits": ([u32, u32], u32),
             "am_matmul_workspace_bytes": ([vp, vp, vp, u32], u64),
             "am_matmul_packed": ([u32, vp, vp, u32, u32, u32, u32, vp, vp, vp, u64, vp], i32),
@@ -320,6 +332,51 @@ def groupby(keys, values, mode=GB_EXACT, value_pred=None, want_counts=True, ws=N
                             ctypes.c_void_p(ocnt.data_ptr()) if ocnt is not None else None,
                             ctypes.c_void_p(res.data_ptr()), ws.ptr(), ws.bytes, _stream(stream)), "am_groupby")
     return ok, osum, ocnt, res
+
+
+class PipelineBatch:
+    """A prepared batch of fused filter -> compressstore(a+b) -> sum chains.
+
+    columns: list of (a Column, b Column, (pred_op, c), s_fmt, s_bits).
+    run() launches one persistent kernel over every tile of every column
+    (capturable in a CUDA/HIP graph); results: per column the output Column
+    s (capacity n) and stats {count, overflow, total, 0}.
+    """
+
+    def __init__(self, columns, variant=16, outs=None):
+        torch = _torch()
+        n = len(columns)
+        self.cols = columns
+        self.variant = variant
+        self.outs = []
+        self.stats = device_words(4 * n)
+        arr = (am_pipeline_col * n)()
+        ns = (ctypes.c_uint64 * n)()
+        for i, (a, b, pred, sfmt, sbits) in enumerate(columns):
+            out = outs[i] if outs is not None else Column(a.n, sbits, sfmt)
+            self.outs.append(out)
+            arr[i] = am_pipeline_col(a.c(), b.c(), out.c(), am_predicate(pred[0], 0, pred[1]),
+                                     self.stats.data_ptr() + 32 * i)
+            ns[i] = a.n
+        self.arr = arr
+        self.in_format = columns[0][0].fmt
+        self.ws_bytes = int(lib().am_pipeline_workspace_bytes(ns, n))
+        self.ws = device_words((self.ws_bytes + 7) // 8)
+        torch.cuda.synchronize()
+        tt = ctypes.c_uint32(0)
+        _check(lib().am_pipeline_prepare(arr, n, variant, ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes,
+                                         ctypes.byref(tt)), "am_pipeline_prepare")
+        self.total_tiles = tt.value
+        self.n = n
+
+    def run(self, stream=None):
+        _check(lib().am_pipeline_run(self.n, self.total_tiles, self.in_format, self.variant,
+                                     ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes, _stream(stream)),
+               "am_pipeline_run")
+
+    def results(self):
+        st = u64(self.stats).reshape(self.n, 4)
+        return self.outs, st
 
 
 def matmul_out_bits(bits, K):

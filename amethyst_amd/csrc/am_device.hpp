@@ -33,6 +33,16 @@ struct Col {
     uint32_t eps;       // SLAB elements per word
     uint32_t eps_magic; // SLAB: floor(2^32/eps)+1, exact quotient for idx < 2^24
     uint32_t pad;
+    uint64_t words;     // uint64 words of the buffer
+};
+
+// A predicate normalised on the host to a branch-free range test over the
+// column's value range: selected = ((v - lo) <= span) != inv.
+struct Pred {
+    uint64_t lo, span;
+    uint32_t inv, pad;
+    __device__ inline bool operator()(uint64_t v) const { return ((v - lo) <= span) != (inv != 0); }
+    __device__ inline bool sel32(uint32_t v) const { return ((v - (uint32_t)lo) <= (uint32_t)span) != (inv != 0); }
 };
 
 // 32-bit quotient idx/eps for idx < 2^24 and eps in 2..64 (multiply-high by
@@ -50,6 +60,7 @@ struct TileReader {
     uint32_t r0;          // SLAB: t0 % eps ; BITSTREAM: (t0*bits) % 64
     uint64_t mask;
     uint32_t bits, cbytes, eps, eps_magic;
+    uint32_t last32; // BITSTREAM: last valid u32 index relative to base (clamp)
     Col c;
 
     __device__ inline TileReader(const Col& col, uint64_t t0) : c(col)
@@ -73,6 +84,8 @@ struct TileReader {
             base = col.p + (bit >> 6);
             r0 = (uint32_t)(bit & 63);
         }
+        const uint64_t rel = col.words - (uint64_t)(base - col.p); // words from base to the end
+        last32 = (uint32_t)(rel * 2 > 0xFFFFFFFFull ? 0xFFFFFFFFull : (rel ? rel * 2 - 1 : 0));
     }
 
     // element t0 + i (caller guarantees it is < n)
@@ -110,13 +123,12 @@ template <int IK, bool WIDE>
 struct Fetch {
     uint64_t raw;
     uint32_t sh;
-    __device__ inline void load(const TileReader<IK>& rd, uint32_t i, bool inb)
+    // Branch-free: callers clamp `i` to a valid element and mask the result.
+    __device__ inline void load(const TileReader<IK>& rd, uint32_t i)
     {
-        raw = 0;
         sh = 0;
-        if (!inb) return;
         if (IK == K_STRIDED) {
-            switch (rd.cbytes) {
+            switch (rd.cbytes) { // uniform
                 case 1: raw = ((const uint8_t*)rd.base)[i]; break;
                 case 2: raw = ((const uint16_t*)rd.base)[i]; break;
                 case 4: raw = ((const uint32_t*)rd.base)[i]; break;
@@ -132,9 +144,9 @@ struct Fetch {
         else if (!WIDE) {
             const uint32_t lb = rd.r0 + i * rd.bits;
             const uint32_t* p32 = (const uint32_t*)rd.base;
-            const uint32_t lo = p32[lb >> 5];
-            const uint32_t hi = ((lb & 31) + rd.bits > 32) ? p32[(lb >> 5) + 1] : 0u;
-            raw = ((uint64_t)hi << 32) | lo;
+            const uint32_t i0 = lb >> 5;
+            const uint32_t i1 = i0 + 1 <= rd.last32 ? i0 + 1 : rd.last32;
+            raw = ((uint64_t)p32[i1] << 32) | p32[i0];
             sh = lb & 31;
         }
         else {

@@ -37,6 +37,7 @@ inline int make_col(const am_column* c, Col* out)
     d.mask = lowmask(c->bits);
     d.eps = 64 / c->bits;
     d.eps_magic = d.eps > 1 ? (uint32_t)((1ull << 32) / d.eps + 1) : 0;
+    d.words = am_packed_words(c->format, c->bits, c->n);
     switch (c->format) {
         case AM_FMT_BASELINE: d.kind = K_STRIDED; d.cbytes = 8; break;
         case AM_FMT_TIGHT: d.kind = K_STRIDED; d.cbytes = tight_bytes(c->bits); break;
@@ -46,6 +47,32 @@ inline int make_col(const am_column* c, Col* out)
     }
     *out = d;
     return AM_OK;
+}
+
+// Normalise `v OP c` over values in [0, vmax] to selected = ((v-lo) <= span) != inv.
+inline Pred make_pred(const am_predicate& p, uint64_t vmax)
+{
+    // the selected set as a closed range [a, b] (or its complement for NE)
+    uint64_t a = 0, b = vmax;
+    bool empty = false, inv = false;
+    switch (p.op) {
+        case AM_PRED_LT: if (p.c == 0) empty = true; else b = p.c - 1 < vmax ? p.c - 1 : vmax; break;
+        case AM_PRED_LE: b = p.c < vmax ? p.c : vmax; break;
+        case AM_PRED_GT: if (p.c >= vmax) empty = true; else a = p.c + 1; break;
+        case AM_PRED_GE: if (p.c > vmax) empty = true; else a = p.c; break;
+        case AM_PRED_EQ: if (p.c > vmax) empty = true; else a = b = p.c; break;
+        default: // NE
+            if (p.c > vmax) break; // everything
+            a = b = p.c;
+            inv = true;
+            break;
+    }
+    Pred r{};
+    if (empty) { r.lo = 0; r.span = ~0ull; r.inv = 1; return r; }
+    r.lo = a;
+    r.span = b - a;
+    r.inv = inv ? 1 : 0;
+    return r;
 }
 
 inline int check_launch()

@@ -41,9 +41,9 @@ constexpr uint32_t CS_TILE = CS_WAVES * CS_J * 64; // 4096
 struct CompressArgs {
     Col in;
     const uint64_t* mask_in;
-    uint64_t pc;
-    uint32_t pop;
-    uint32_t T; // tiles
+    Pred pr;
+    uint32_t chk_ovf; // output narrower than input: count values that do not fit
+    uint32_t T;       // tiles
     Col out;
     uint64_t cap; // output capacity in elements
     uint64_t* mask_out;
@@ -102,9 +102,19 @@ __global__ __launch_bounds__(256) void k_compress(CompressArgs a)
     uint64_t mk = 0; // lane j < CS_J: mask word j of this wave's chunk (mask mode)
     auto fetch_tile = [&](uint32_t tt, Fetch<IK, WIDE>* ff, uint64_t& mm) {
         const uint64_t wb = (uint64_t)tt * CS_TILE + wave * (CS_J * 64);
-        TileReader<IK> rd(a.in, wb < n ? wb : 0);
+        if (wb < n) {
+            const uint32_t rem = (uint32_t)(n - wb < CS_J * 64 ? n - wb : CS_J * 64);
+            TileReader<IK> rd(a.in, wb);
 #pragma unroll
-        for (uint32_t j = 0; j < CS_J; j++) ff[j].load(rd, j * 64 + lane, wb + j * 64 + lane < n);
+            for (uint32_t j = 0; j < CS_J; j++) {
+                const uint32_t i = j * 64 + lane;
+                ff[j].load(rd, i < rem ? i : rem - 1);
+            }
+        }
+        else {
+#pragma unroll
+            for (uint32_t j = 0; j < CS_J; j++) ff[j].raw = 0, ff[j].sh = 0;
+        }
         if (!PRED) {
             const uint64_t w = (wb >> 6) + lane;
             mm = (lane < CS_J && w < ((n + 63) >> 6)) ? a.mask_in[w] : 0;
@@ -121,15 +131,16 @@ __global__ __launch_bounds__(256) void k_compress(CompressArgs a)
 
         // ---- 1. decode, select, count (lane j keeps ballot j in `mine`)
         const uint64_t wbase = (uint64_t)t * CS_TILE + wave * (CS_J * 64);
+        const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < CS_J * 64 ? n - wbase : CS_J * 64) : 0;
         uint64_t mine = 0;
         uint32_t cnt = 0, ovf = 0;
 #pragma unroll
         for (uint32_t j = 0; j < CS_J; j++) {
-            const bool inb = wbase + j * 64 + lane < n;
+            const bool inb = j * 64 + lane < wrem;
             const uint64_t v = f[j].value(a.in.mask);
             bool fl;
             if (PRED) {
-                fl = inb && pred_eval(a.pop, a.pc, v);
+                fl = inb && (WIDE ? a.pr(v) : a.pr.sel32((uint32_t)v));
             }
             else {
                 const uint64_t mw = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mk >> 32), j) << 32) |
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(256) void k_compress(CompressArgs a)
             const uint64_t b = ballot(fl);
             if (lane == j) mine = b;
             cnt += (uint32_t)__popcll(b);
-            ovf += (uint32_t)__popcll(ballot(fl && (v & ~om) != 0));
+            if (a.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && (v & ~om) != 0));
         }
         if (a.mask_out) {
             const uint64_t w = (wbase >> 6) + lane;
@@ -348,8 +359,8 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     CompressArgs a{};
     a.in = ci;
     a.mask_in = mask;
-    a.pc = pred ? pred->c : 0;
-    a.pop = pred ? pred->op : 0;
+    if (pred) a.pr = make_pred(*pred, ci.mask);
+    a.chk_ovf = co.bits < ci.bits ? 1 : 0;
     a.T = (uint32_t)T;
     a.out = co;
     a.cap = out->n;

@@ -27,7 +27,7 @@ constexpr uint32_t FT_TILE = 4 * FT_J * 64;   // elements per 256-thread tile
 // Persistent: block b filters tiles b, b+G, ... and loads tile t+G while it
 // evaluates tile t, so every wave keeps a tile's worth of loads in flight.
 template <int IK, bool WIDE>
-__global__ __launch_bounds__(256) void k_filter(Col in, uint32_t pop, uint64_t pc, uint64_t* mask_out, uint32_t T)
+__global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_out, uint32_t T)
 {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t n = in.n, words = (n + 63) >> 6;
@@ -36,19 +36,27 @@ __global__ __launch_bounds__(256) void k_filter(Col in, uint32_t pop, uint64_t p
     Fetch<IK, WIDE> f[FT_J];
     auto fetch_tile = [&](uint32_t tt, Fetch<IK, WIDE>* ff) {
         const uint64_t wb = (uint64_t)tt * FT_TILE + wave * (FT_J * 64);
-        TileReader<IK> rd(in, wb < n ? wb : 0);
+        if (wb < n) {
+            const uint32_t rem = (uint32_t)(n - wb < FT_J * 64 ? n - wb : FT_J * 64);
+            TileReader<IK> rd(in, wb);
 #pragma unroll
-        for (uint32_t j = 0; j < FT_J; j++) ff[j].load(rd, j * 64 + lane, wb + j * 64 + lane < n);
+            for (uint32_t j = 0; j < FT_J; j++) {
+                const uint32_t i = j * 64 + lane;
+                ff[j].load(rd, i < rem ? i : rem - 1);
+            }
+        }
     };
     fetch_tile(t, f);
     for (; t < T; t += gridDim.x) {
         Fetch<IK, WIDE> nf[FT_J];
         if (t + gridDim.x < T) fetch_tile(t + gridDim.x, nf);
         const uint64_t wbase = (uint64_t)t * FT_TILE + wave * (FT_J * 64);
+        const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < FT_J * 64 ? n - wbase : FT_J * 64) : 0;
         uint64_t mine = 0;
 #pragma unroll
         for (uint32_t j = 0; j < FT_J; j++) {
-            const bool fl = (wbase + j * 64 + lane < n) && pred_eval(pop, pc, f[j].value(in.mask));
+            const uint64_t v = f[j].value(in.mask);
+            const bool fl = (j * 64 + lane < wrem) && (WIDE ? pr(v) : pr.sel32((uint32_t)v));
             const uint64_t b = ballot(fl);
             if (lane == j) mine = b;
         }
@@ -69,7 +77,12 @@ __device__ inline uint64_t slab_word_sum(uint64_t w, uint64_t q, const Col& c)
     uint32_t cnt = c.eps;
     if (first + cnt > c.n) cnt = (uint32_t)(c.n - first);
     uint64_t s = 0;
-    if (c.bits <= 32) {
+    if (c.bits == 32) {
+        // v_bfe encodes the width in 5 bits: a 32-bit field is the half itself
+        s = (uint32_t)w;
+        if (cnt > 1) s += (uint32_t)(w >> 32);
+    }
+    else if (c.bits < 32) {
         const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
         for (uint32_t k = 0; k < cnt; k++) {
             const uint32_t off = k * c.bits;
@@ -179,13 +192,14 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
     if (c.n == 0) return AM_OK;
     const uint64_t T = (c.n + FT_TILE - 1) / FT_TILE;
     if (T >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
+    const Pred pr = make_pred(pred, c.mask);
     hipStream_t s = (hipStream_t)stream;
 #define FT_GO(IK, W)                                                                                        \
     {                                                                                                       \
         static uint32_t capb = 0;                                                                           \
         if (!capb) capb = resident_blocks(k_filter<IK, W>, 256, 0);                                         \
         const uint32_t g = (uint32_t)(T < capb ? T : capb);                                                 \
-        hipLaunchKernelGGL((k_filter<IK, W>), dim3(g), dim3(256), 0, s, c, pred.op, pred.c, mask_out, (uint32_t)T); \
+        hipLaunchKernelGGL((k_filter<IK, W>), dim3(g), dim3(256), 0, s, c, pr, mask_out, (uint32_t)T); \
     }
     const bool wide = c.bits > 32;
     switch (c.kind) {

@@ -167,6 +167,53 @@ __global__ __launch_bounds__(256) void k_gen_column(uint64_t s0, Col out, CdfMap
     out.p[gid] = word;
 }
 
+// Sorted with noise: v_i = min(mask, floor(i * 2^bits / n) + (draw(i) & noise_mask)).
+template <int OK>
+__global__ __launch_bounds__(256) void k_gen_sorted(uint64_t s0, Col out, uint64_t noise_mask)
+{
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    auto val = [&](uint64_t e, uint32_t d) -> uint64_t {
+        const double frac = (double)e / (double)out.n;
+        uint64_t base = (uint64_t)(frac * (double)(out.mask)) ;
+        uint64_t v = base + (d & noise_mask);
+        return v > out.mask ? out.mask : v;
+    };
+    if (OK == K_STRIDED) {
+        if (gid >= out.n) return;
+        const uint64_t v = val(gid, pcg_out(state_at(s0, gid)));
+        switch (out.cbytes) {
+            case 1: ((uint8_t*)out.p)[gid] = (uint8_t)v; break;
+            case 2: ((uint16_t*)out.p)[gid] = (uint16_t)v; break;
+            case 4: ((uint32_t*)out.p)[gid] = (uint32_t)v; break;
+            default: out.p[gid] = v;
+        }
+        return;
+    }
+    const uint64_t words = OK == K_SLAB ? (out.n + out.eps - 1) / out.eps : (out.n * out.bits + 63) / 64;
+    if (gid >= words) return;
+    uint64_t e0, e1;
+    if (OK == K_SLAB) {
+        e0 = gid * out.eps;
+        e1 = e0 + out.eps;
+    }
+    else {
+        e0 = (gid * 64) / out.bits;
+        e1 = (gid * 64 + 63) / out.bits + 1;
+    }
+    if (e1 > out.n) e1 = out.n;
+    uint64_t st = state_at(s0, e0), word = 0;
+    for (uint64_t e = e0; e < e1; e++) {
+        const uint64_t v = val(e, pcg_out(st));
+        st = PCG_MULT * st + PCG_INC;
+        if (OK == K_SLAB) word |= v << ((e - e0) * out.bits);
+        else {
+            const int64_t pos = (int64_t)(e * out.bits) - (int64_t)(gid * 64);
+            word |= pos >= 0 ? (v << pos) : (v >> (-pos));
+        }
+    }
+    out.p[gid] = word;
+}
+
 // Bernoulli mask: bit i = draw(i) < thr (thr = round(s * 2^32)); lane per word.
 __global__ __launch_bounds__(256) void k_gen_mask_bernoulli(uint64_t s0, uint64_t thr, uint64_t* mask, uint64_t n)
 {
@@ -262,6 +309,25 @@ extern "C" int am_gen_from_cdf(uint64_t seed, const uint32_t* thresholds_dev, ui
 {
     if (!thresholds_dev || nbins == 0) return AM_E_INVALID;
     return gen_column(seed, out, thresholds_dev, nbins, stream);
+}
+
+extern "C" int am_gen_sorted_noise(uint64_t seed, uint32_t noise_bits, am_column* out, void* stream)
+{
+    Col c;
+    int st = make_col(out, &c);
+    if (st) return st;
+    if (c.n == 0) return AM_OK;
+    const uint64_t units = c.kind == K_STRIDED ? c.n : am_packed_words(out->format, out->bits, out->n);
+    const uint32_t blocks = (uint32_t)((units + 255) / 256);
+    const uint64_t nm = lowmask(noise_bits > 32 ? 32 : noise_bits) & (noise_bits ? ~0ull : 0ull);
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t s0 = splitmix_seed(seed);
+    switch (c.kind) {
+        case K_STRIDED: hipLaunchKernelGGL(k_gen_sorted<K_STRIDED>, dim3(blocks), dim3(256), 0, s, s0, c, nm); break;
+        case K_SLAB: hipLaunchKernelGGL(k_gen_sorted<K_SLAB>, dim3(blocks), dim3(256), 0, s, s0, c, nm); break;
+        default: hipLaunchKernelGGL(k_gen_sorted<K_BITSTREAM>, dim3(blocks), dim3(256), 0, s, s0, c, nm); break;
+    }
+    return check_launch();
 }
 
 extern "C" int am_gen_mask_bernoulli(uint64_t seed, double selectivity, uint64_t* mask, uint64_t n, void* stream)

@@ -28,8 +28,8 @@ constexpr uint32_t GB_THREADS = 1024;
 
 struct GroupArgs {
     Col keys, vals;
-    uint64_t pc;
-    uint32_t pop, use_pred;
+    Pred pr;
+    uint32_t use_pred, pad;
     uint32_t kbits, mode;
     uint32_t SB;          // sum field bits of the packed LDS word
     uint32_t fold_tiles;  // tiles between packed -> wide folds
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(GB_THREADS) void k_groupby_partial(GroupArgs a)
 #pragma unroll
         for (uint32_t r = 0; r < GB_TILE / GB_THREADS; r++) {
             const uint32_t i = r * GB_THREADS + tid;
-            if (i < rem && (!a.use_pred || pred_eval(a.pop, a.pc, vv[r])))
+            if (i < rem && (!a.use_pred || a.pr(vv[r])))
                 atomicAdd((unsigned long long*)&packed[kv[r]], (unsigned long long)(ONE | vv[r]));
         }
         if (++since == a.fold_tiles) {
@@ -233,8 +233,7 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     a.keys = ck;
     a.vals = cv;
     a.use_pred = value_pred ? 1 : 0;
-    a.pop = value_pred ? value_pred->op : 0;
-    a.pc = value_pred ? value_pred->c : 0;
+    if (value_pred) a.pr = make_pred(*value_pred, cv.mask);
     a.kbits = ck.bits;
     a.mode = mode;
     // packed word: sum field SB bits, count field 64-SB bits; R = 2^r rows per fold

@@ -160,6 +160,11 @@ int am_gen_uniform(uint64_t seed, am_column* out, void* stream);
  * used for Zipf-distributed group-by keys. No reference counterpart. */
 int am_gen_from_cdf(uint64_t seed, const uint32_t* thresholds_dev, uint32_t nbins, am_column* out, void* stream);
 
+/* Ascending values with noise: v_i = min(mask, floor(i*mask/n) + (draw(i) &
+ * (2^noise_bits - 1))) -- the "sorted with noise" columns of the uneven
+ * batch workload. No reference counterpart. */
+int am_gen_sorted_noise(uint64_t seed, uint32_t noise_bits, am_column* out, void* stream);
+
 /* Bernoulli(selectivity) mask: bit i = draw(i) < round(s * 2^32). LSB-first
  * counterpart of generate_mask_uniform (src/main.cu:33-46), which writes
  * MSB-first bytes from a serial stream. */
@@ -231,6 +236,55 @@ int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t ws_bytes, 
 int am_groupby(const am_column* keys, const am_column* values, const am_predicate* value_pred,
                uint32_t mode, uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts,
                uint64_t* result_dev, void* ws, uint64_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------- batched operator chain */
+
+/* One column of a batch: mask = filter(a, pred); s = compressstore(a, mask)
+ * + compressstore(b, mask) (add, width bits+1); total = sum(s). `s` gives the
+ * output format (slab or bitstream), width and capacity (s.n); stats_dev
+ * (device, 4 x uint64) receives {selected count, values that did not fit
+ * s.bits, total, 0}. a and b: same n, bits (<= 31) and format. */
+typedef struct {
+    am_column a;
+    am_column b;
+    am_column s;
+    am_predicate pred;
+    uint64_t* stats;
+} am_pipeline_col;
+
+/* Workspace for a batch whose columns have lengths n[0..count). */
+uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t count);
+
+/* Validate the batch and upload its job table into the workspace
+ * (synchronous); *total_tiles receives the tile count to pass to run.
+ * variant = elements per lane per tile (8 or 16: 2048- or 4096-element
+ * tiles). The reference chains operators only by format, one launch each
+ * (src/op_chaining_demo.cu:137-152); here the chain is fused. */
+int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, uint32_t variant, void* ws, uint64_t ws_bytes,
+                        uint32_t* total_tiles);
+
+/* Run a prepared batch: one persistent launch over every tile of every
+ * column plus a one-block epoch update; no host work, graph-capturable. */
+int am_pipeline_run(uint32_t count, uint32_t total_tiles, uint32_t in_format, uint32_t variant, void* ws,
+                    uint64_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------- learned planner */
+
+/* 32 fp32 features per column of a batch (width, length, sampled
+ * selectivity, 16-bin histogram, sampled max of selected a+b, sortedness,
+ * selection run length, byte estimates) into feat_dev[count][32]. scratch_dev
+ * holds am_plan_scratch_bytes(count) bytes. No reference counterpart. */
+uint64_t am_plan_scratch_bytes(uint32_t count);
+int am_plan_features(const am_pipeline_col* cols, uint32_t count, float* feat_dev, void* scratch_dev, void* stream);
+
+/* MLP inference (32 -> 64 -> 64 -> nplans, ReLU, fp32) over normalised
+ * features; plan_dev[c] = argmin of the predicted relative log-times, or
+ * fallback_dev[c] when the best beats the runner-up by less than min_margin
+ * (fallback_dev may be NULL). score_dev (may be NULL) receives the
+ * predictions. Weights: W1[64][32] b1[64] W2[64][64] b2[64] W3[P][64] b3[P]. */
+int am_plan_infer(const float* feat_dev, uint32_t count, const float* weights_dev, uint32_t nplans,
+                  const float* fmean_dev, const float* fstd_dev, float min_margin, const int32_t* fallback_dev,
+                  int32_t* plan_dev, float* score_dev, void* stream);
 
 /* ---------------------------------------------------------------- packed matmul */
 

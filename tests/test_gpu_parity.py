@@ -295,3 +295,89 @@ def test_matmul_uneven_batch_packed_only(am, oracle_mod):
     for (cp, _, ob), C in zip(outs, exps):
         exp_words = _pack_matrix(oracle_mod, C.astype(np.uint64), ob)
         assert np.array_equal(am.u64(cp)[:len(exp_words)], exp_words)
+
+
+def test_pipeline_batch(am, oracle_mod):
+    """Fused filter -> compressstore(a+b) -> sum over a batch of uneven columns."""
+    rng = np.random.default_rng(12)
+    cols, refs = [], []
+    for i in range(24):
+        n = int([1, 63, 4096, 4097, 30011, 200003][i % 6] * (1 + (i // 6) % 2))
+        w = int(rng.integers(3, 32))
+        kind = i % 3
+        if kind == 0:
+            a = rand_vals(rng, n, w)
+        elif kind == 1:
+            g = min(1 << w, 4096)
+            p = 1.0 / np.arange(1, g + 1) ** 1.2
+            a = rng.choice(g, size=n, p=p / p.sum()).astype(np.uint64)
+        else:
+            a = np.minimum((np.arange(n) * ((1 << w) - 1) // max(n, 1)) + rng.integers(0, 8, n), (1 << w) - 1)
+            a = a.astype(np.uint64)
+        b = rand_vals(rng, n, w)
+        c = int(rng.integers(0, 1 << w))
+        fmt = (1, 2)[i % 2]
+        sb = w + 1 if i % 4 else max(1, w - 1)  # some columns too narrow: overflow must be reported
+        cols.append((upload(am, oracle_mod, a, w, 1), upload(am, oracle_mod, b, w, 1), (0, c), fmt, sb))
+        v = (a + b) & np.uint64((1 << (w + 1)) - 1)
+        sel = a < np.uint64(c)
+        refs.append((v, sel, fmt, sb))
+    say("pipeline batch", len(cols))
+    batch = am.PipelineBatch(cols)
+    for rep in range(3):  # repeated runs on one workspace (epoch tags)
+        batch.run()
+    import torch
+    torch.cuda.synchronize()
+    outs, st = batch.results()
+    for i, ((v, sel, fmt, sb), out) in enumerate(zip(refs, outs)):
+        n = len(v)
+        mask = np.packbits(sel, bitorder="little")
+        mask = np.concatenate([mask, np.zeros((-len(mask)) % 8, np.uint8)]).view(np.uint64)
+        k, exp_words, ovf = oracle_mod.compressstore(0, 64, v, n, mask, fmt, sb)
+        total = int(v[sel].sum(dtype=np.uint64))
+        assert [int(x) for x in st[i][:3]] == [k, ovf, total], (i, list(st[i]), k, ovf, total)
+        used = oracle_mod.packed_words(fmt, sb, k)
+        assert np.array_equal(out.host_words()[:used], exp_words[:used]), i
+
+
+def test_planner_kernels(am, oracle_mod, tmp_path):
+    """k_plan_stats features are sane and k_plan_mlp equals a numpy forward pass."""
+    import torch
+    from amethyst_amd import planner
+    cols = []
+    for seed in range(6):
+        spec = planner.column_spec(seed, nmax=1 << 18)
+        cols.append(planner.make_inputs(seed, spec))
+    f = planner.features(cols)
+    fr = f.cpu().numpy()
+    for (a, b, pred), row in zip(cols, fr):
+        assert abs(row[0] - a.bits / 32.0) < 1e-6
+        assert 0.0 <= row[2] <= 1.0 and abs(row[3:19].sum() - 1.0) < 1e-4
+    rng = np.random.default_rng(0)
+    P = planner.NPLANS
+    nw = 64 * 32 + 64 + 64 * 64 + 64 + P * 64 + P
+    z = {"weights": (rng.standard_normal(nw) * 0.2).astype(np.float32), "mean": fr.mean(0).astype(np.float32),
+         "std": (fr.std(0) + 1e-3).astype(np.float32), "nplans": np.int32(P)}
+    path = tmp_path / "w.npz"
+    np.savez(path, **z)
+    pl = planner.Planner(str(path), min_margin=0.0)
+    choice, sc = pl.choose(f, scores=True)
+    ref = planner.mlp_reference(np.load(path), fr)
+    assert np.allclose(sc, ref, rtol=1e-4, atol=1e-4)
+    assert np.array_equal(choice, ref.argmin(axis=1))
+    # an impossible margin sends every column to the analytic plan
+    pl.min_margin = 1e9
+    choice2, _ = pl.choose(f)
+    assert (choice2 == planner.ANALYTIC).all()
+
+
+def test_plain_chain_matches_fused(am, oracle_mod):
+    """The reference-style plain chain and the fused batch agree (totals)."""
+    import torch
+    from amethyst_amd import planner
+    cols = [planner.make_inputs(s, planner.column_spec(s, nmax=1 << 17)) for s in range(7)]
+    plain = planner.run_plain(cols)
+    f = planner.features(cols).cpu().numpy()
+    for p in range(planner.NPLANS):
+        outs, stats, _, _ = planner.run_batch(cols, [p] * len(cols), f)
+        assert [int(s[2]) for s in stats] == plain, p
