@@ -87,6 +87,7 @@ def lib():
             "am_gen_bitmask": ([u32], u64),
             "am_workspace_bytes": ([u64], u64),
             "am_workspace_init": ([vp, u64, vp], i32),
+            "am_workspace_status": ([vp], i32),
             "am_prng_fill_u32": ([u64, u64, vp, u64, vp], i32),
             "am_gen_uniform": ([u64, colp, vp], i32),
             "am_gen_from_cdf": ([u64, vp, u32, colp, vp], i32),
@@ -209,6 +210,11 @@ class Workspace:
 
     def ptr(self):
         return ctypes.c_void_p(self.buf.data_ptr())
+
+    def check(self):
+        """Raise if a single-pass launch on this workspace gave up waiting
+        (bounded look-back wait); synchronous."""
+        _check(lib().am_workspace_status(self.ptr()), "am_workspace_status")
 
 
 # ----------------------------------------------------------------- operators

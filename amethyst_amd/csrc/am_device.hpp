@@ -122,11 +122,9 @@ struct TileReader {
 template <int IK, bool WIDE>
 struct Fetch {
     uint64_t raw;
-    uint32_t sh;
     // Branch-free: callers clamp `i` to a valid element and mask the result.
     __device__ inline void load(const TileReader<IK>& rd, uint32_t i)
     {
-        sh = 0;
         if (IK == K_STRIDED) {
             switch (rd.cbytes) { // uniform
                 case 1: raw = ((const uint8_t*)rd.base)[i]; break;
@@ -139,7 +137,6 @@ struct Fetch {
             const uint32_t idx = rd.r0 + i;
             const uint32_t q = rd.eps == 1 ? idx : __umulhi(idx, rd.eps_magic);
             raw = rd.base[q];
-            sh = (idx - q * rd.eps) * rd.bits;
         }
         else if (!WIDE) {
             const uint32_t lb = rd.r0 + i * rd.bits;
@@ -147,13 +144,22 @@ struct Fetch {
             const uint32_t i0 = lb >> 5;
             const uint32_t i1 = i0 + 1 <= rd.last32 ? i0 + 1 : rd.last32;
             raw = ((uint64_t)p32[i1] << 32) | p32[i0];
-            sh = lb & 31;
         }
         else {
             raw = rd.get(i);
         }
     }
-    __device__ inline uint64_t value(uint64_t mask) const { return (raw >> sh) & mask; }
+    // the same (clamped) element index as load(); the shift is recomputed
+    __device__ inline uint64_t value(const TileReader<IK>& rd, uint32_t i) const
+    {
+        if (IK == K_SLAB) {
+            const uint32_t idx = rd.r0 + i;
+            const uint32_t q = rd.eps == 1 ? idx : __umulhi(idx, rd.eps_magic);
+            return (raw >> ((idx - q * rd.eps) * rd.bits)) & rd.mask;
+        }
+        if (IK == K_BITSTREAM && !WIDE) return (raw >> ((rd.r0 + i * rd.bits) & 31)) & rd.mask;
+        return raw & rd.mask;
+    }
 };
 
 // ------------------------------------------------------------ wave64 helpers
@@ -243,7 +249,8 @@ __device__ inline void drain() { __builtin_amdgcn_s_waitcnt(0); }
 // the group-by partial tables, then the look-back descriptors.
 struct WsHeader {
     uint32_t epoch; // launch counter of the look-back kernels
-    uint32_t pad0[63];
+    uint32_t abort; // nonzero once a look-back wait gave up (poisoned until re-zeroed)
+    uint32_t pad0[62];
 };
 static_assert(sizeof(WsHeader) == 256, "header size");
 
@@ -252,42 +259,50 @@ constexpr uint32_t GB_MAX_BLOCKS = 256;                      // group-by partial
 constexpr uint64_t GB_MAX_KEYS = 4096;                       // keys <= 12 bits
 constexpr uint64_t WS_GBPART_BYTES = GB_MAX_BLOCKS * 2 * GB_MAX_KEYS * 8; // 16 MiB
 constexpr uint64_t WS_FIXED = 256 + WS_SMALL_BYTES + WS_GBPART_BYTES;
-constexpr uint64_t WS_DESC_BYTES_PER_TILE = 5 * 8; // status + 2 window words per level
 
 __host__ __device__ inline uint64_t* ws_small(void* ws) { return (uint64_t*)((uint8_t*)ws + 256); }
 __host__ __device__ inline uint64_t* ws_gbpart(void* ws) { return (uint64_t*)((uint8_t*)ws + 256 + WS_SMALL_BYTES); }
 
-// Look-back descriptors. Every word carries the launch epoch, so stale words
-// from earlier launches are recognised without ever clearing the arrays:
-//   status[t]    [63:62] level (1 aggregate, 2 inclusive) [61:40] epoch&(2^22-1) [39:0] count
-//   winA/winP[2t..2t+1]  [63:48] epoch&0xffff [47:46] level [45:0] payload
-// The two payload words hold a 63-bit window (46 + 17 bits) and a 29-bit
-// saturating overflow count. Each launch also zeroes one slice of the
-// indices it does not use, so no word can outlive 2^16 launches unrewritten.
-struct Lookback {
-    uint64_t* status; // [cap]
-    uint64_t* winA;   // [2*cap]
-    uint64_t* winP;   // [2*cap]
-    uint64_t cap;
-};
-
-__host__ __device__ inline Lookback lookback_of(void* ws, uint64_t ws_bytes)
-{
-    Lookback L;
-    uint64_t cap = ws_bytes > WS_FIXED ? (ws_bytes - WS_FIXED) / WS_DESC_BYTES_PER_TILE : 0;
-    uint64_t* d = (uint64_t*)((uint8_t*)ws + WS_FIXED);
-    L.status = d;
-    L.winA = d + cap;
-    L.winP = d + 3 * cap;
-    L.cap = cap;
-    return L;
-}
-
+// ------------------------------------------------------------ two-level look-back
+//
+// Scan element of the single-pass compaction kernels: (count, window,
+// overflow[, sum]). The window is the last 63 bits of the dense stream of
+// selected values at the output width, right-aligned (bit 62 = last stream
+// bit); appending y (m = y.count*bits stream bits) to x gives
+// (x.win >> m) | y.win when m < 63, else y.win -- so the tile that completes
+// an output word can build it from its predecessors' values.
+//
+// One block per tile, tiles in launch order: a tile waits only on lower
+// tiles, whose blocks were dispatched before it, so the look-back always
+// makes progress without a tile counter.
+//
+// Descriptors carry the launch epoch so stale words from earlier launches
+// are recognised without clearing: a status word
+//   [63:62] level (1 aggregate, 2 inclusive) [61:40] epoch&(2^22-1) [39:0] count
+// and, when the scan carries more than a count, payload words
+//   [63:48] epoch&0xffff [47:46] level [45:0] payload
+// (window low 46 bits; window high 17 bits | overflow << 17; sum low 46
+// bits; sum high 18 bits). Per tile: status, aggregate words, inclusive
+// words. Per group of 64 tiles: an aggregate of the whole group, published
+// by the group's last tile when its own walk over its 63 group-mates found
+// no inclusive value (otherwise that tile's inclusive value arrives just as
+// soon). A look-back therefore needs two dependent round trips: the
+// group-mates below the tile, then whole groups, 64 per step, stopping at
+// the nearest tile or group end whose inclusive value is known.
+//
+// Every wait is bounded: after LB_WAIT_TICKS of the 100 MHz wall clock the
+// waiting wave sets the workspace's abort word and all waiters stop. An
+// aborted workspace stays poisoned (launches that see it do nothing except
+// marking their statistics invalid) until it is zero-filled again.
 constexpr uint32_t LB_CLEAR_SLICES = 64;
 constexpr uint64_t CNT_MASK = (1ull << 40) - 1;
 constexpr uint64_t P46 = (1ull << 46) - 1;
 constexpr uint64_t OVF_MAX = (1ull << 29) - 1;
 constexpr uint64_t WIN_BITS = (1ull << 63) - 1;
+constexpr uint64_t LB_WAIT_TICKS = 200000000ull; // 2 s at 100 MHz
+constexpr uint64_t STATS_INVALID = ~0ull;        // statistics of an aborted launch
+
+enum LbFlags : uint32_t { LB_WIN = 1, LB_OVF = 2, LB_SUM = 4 };
 
 __device__ inline uint64_t lb_status(uint32_t level, uint32_t epoch, uint64_t count)
 {
@@ -305,11 +320,6 @@ __device__ inline bool lb_word_ok(uint64_t w, uint32_t level, uint32_t epoch)
 {
     return (w >> 48) == (epoch & 0xFFFF) && ((w >> 46) & 3) == level;
 }
-
-// Scan element of the compaction: (count, window, overflow). The window is
-// the last 63 bits of the dense stream of selected values at the output
-// width, right-aligned (bit 62 = last stream bit). Appending y
-// (m = y.count*bits stream bits) to x: (x.win >> m) | y.win if m < 63, else y.win.
 __device__ inline uint64_t win_combine(uint64_t xw, uint64_t yw, uint64_t ycount, uint32_t bits)
 {
     uint64_t m = ycount * bits;
@@ -321,5 +331,243 @@ __device__ inline uint64_t ovf_add(uint64_t a, uint64_t b)
     uint64_t s = a + b;
     return s > OVF_MAX ? OVF_MAX : s;
 }
+
+struct Agg {
+    uint64_t c, w, o, s;
+};
+
+// x (older) then y (newer)
+__device__ inline Agg agg_combine(const Agg& x, const Agg& y, uint32_t bits)
+{
+    Agg r;
+    r.c = x.c + y.c;
+    r.w = win_combine(x.w, y.w, y.c, bits);
+    r.o = ovf_add(x.o, y.o);
+    r.s = x.s + y.s;
+    return r;
+}
+
+__device__ inline Agg shfl_agg(const Agg& a, int m)
+{
+    return Agg{shfl_xor_u64(a.c, m), shfl_xor_u64(a.w, m), shfl_xor_u64(a.o, m), shfl_xor_u64(a.s, m)};
+}
+
+// Ordered wave reduction: lane j holds element j, higher lanes are OLDER;
+// every lane returns the combination oldest..newest.
+__device__ inline Agg wave_combine_ordered(Agg v, uint32_t lane, uint32_t bits)
+{
+    for (int m = 1; m < 64; m <<= 1) {
+        const Agg o = shfl_agg(v, m);
+        v = (lane & m) ? agg_combine(v, o, bits) : agg_combine(o, v, bits);
+    }
+    return v;
+}
+
+template <uint32_t F>
+struct Lookback {
+    static constexpr uint32_t NX = ((F & (LB_WIN | LB_OVF)) ? 2 : 0) + ((F & LB_SUM) ? 2 : 0); // payload words
+    static constexpr uint32_t NXA = NX ? NX : 1;                                           // array size
+    static constexpr uint32_t TS = 1 + 2 * NX; // tile stride: status, aggregate words, inclusive words
+    static constexpr uint32_t GS = 1 + NX;     // group stride: status, aggregate words
+    uint64_t* tiles;  // [cap][TS]
+    uint64_t* groups; // [cap/64 + 2][GS]
+    uint64_t cap;     // tiles
+    uint32_t epoch, bits;
+    WsHeader* hdr;
+
+    __host__ __device__ static uint64_t bytes_for(uint64_t tiles_cap)
+    {
+        return tiles_cap * TS * 8 + (tiles_cap / 64 + 2) * GS * 8;
+    }
+    __host__ __device__ static Lookback at(void* base, uint64_t bytes)
+    {
+        Lookback L{};
+        const uint64_t fixed = 2 * GS * 8;
+        uint64_t c = bytes > fixed ? ((bytes - fixed) * 64) / (TS * 8 * 64 + GS * 8) : 0;
+        while (c && bytes_for(c) > bytes) c--;
+        while (bytes_for(c + 1) <= bytes) c++;
+        L.cap = c;
+        L.tiles = (uint64_t*)base;
+        L.groups = L.tiles + L.cap * TS;
+        return L;
+    }
+    __device__ uint64_t* tile(uint64_t t) const { return tiles + t * TS; }
+    __device__ uint64_t* group(uint64_t g) const { return groups + g * GS; }
+
+    // ---- bounded waiting
+    __device__ bool aborted() const { return ld32(&hdr->abort) != 0; }
+    // one round of a wait loop (wave-uniform); false once the launch gave up
+    __device__ bool keep_waiting(uint32_t& it, uint64_t& t0) const
+    {
+        if (it++ == 0) t0 = wall_clock64();
+        __builtin_amdgcn_s_sleep(2);
+        if ((it & 63) == 0) {
+            if (aborted()) return false;
+            if (wall_clock64() - t0 > LB_WAIT_TICKS) {
+                st32(&hdr->abort, 1u);
+                return false;
+            }
+        }
+        return true;
+    }
+
+    // ---- payload words
+    __device__ void put_x(uint64_t* p, uint32_t level, const Agg& a) const
+    {
+        uint32_t k = 0;
+        if (F & (LB_WIN | LB_OVF)) {
+            st_rel(&p[k++], lb_word(level, epoch, a.w));
+            st_rel(&p[k++], lb_word(level, epoch, (a.w >> 46) | (a.o << 17)));
+        }
+        if (F & LB_SUM) {
+            st_rel(&p[k++], lb_word(level, epoch, a.s));
+            st_rel(&p[k++], lb_word(level, epoch, a.s >> 46));
+        }
+    }
+    __device__ void load_x(const uint64_t* p, uint64_t* x) const
+    {
+        for (uint32_t k = 0; k < NX; k++) x[k] = ld_acq(&p[k]);
+    }
+    // decode payload words (true when every tag is current)
+    __device__ bool decode(const uint64_t* x, uint32_t level, uint64_t count, Agg& a) const
+    {
+        bool ok = true;
+        for (uint32_t k = 0; k < NX; k++) ok = ok && lb_word_ok(x[k], level, epoch);
+        a.c = count;
+        a.w = 0;
+        a.o = 0;
+        a.s = 0;
+        uint32_t k = 0;
+        if (F & (LB_WIN | LB_OVF)) {
+            a.w = (x[0] & P46) | ((x[1] & 0x1FFFF) << 46);
+            a.o = (x[1] & P46) >> 17;
+            k = 2;
+        }
+        if (F & LB_SUM) a.s = (x[k] & P46) | ((x[k + 1] & P46) << 46);
+        return ok;
+    }
+
+    // Wave-level: publish tile t's aggregate, look back, publish its
+    // inclusive value. Returns the exclusive prefix (all lanes).
+    __device__ Agg run(uint64_t t, const Agg& own, uint32_t lane) const
+    {
+        const Agg zero{0, 0, 0, 0};
+        uint64_t* D = tile(t);
+        if (lane == 0) {
+            const uint32_t lvl = t == 0 ? 2 : 1;
+            if (NX) put_x(D + 1 + (lvl - 1) * NX, lvl, own);
+            st_rel(&D[0], lb_status(lvl, epoch, own.c));
+        }
+        if (t == 0) return zero;
+        const uint64_t g = t >> 6;
+        const uint32_t below = (uint32_t)(t & 63);
+        Agg excl = zero;
+        bool done = false;
+        bool gave_up = false;
+        if (below) {
+            // stage 1: lane j -> tile t-1-j (higher lanes older)
+            const bool act = lane < below;
+            const uint64_t* q = tile(act ? t - 1 - lane : t);
+            uint64_t st = 0, xa[NXA], xp[NXA];
+            if (act) {
+                st = ld_acq(q);
+                load_x(q + 1, xa);
+                load_x(q + 1 + NX, xp);
+            }
+            uint32_t it = 0, stop = below;
+            uint64_t t0 = 0, p2 = 0;
+            Agg v = zero;
+            for (;;) {
+                const bool sok = act && lb_status_ok(st, epoch);
+                p2 = ballot(sok && (st >> 62) == 2);
+                stop = p2 ? (uint32_t)__builtin_ctzll(p2) : below;
+                const bool need = lane < stop || (p2 && lane == stop);
+                bool ok = true;
+                if (need) ok = sok && decode(lane < stop ? xa : xp, lane < stop ? 1 : 2, st & CNT_MASK, v);
+                if (!ballot(need && !ok)) break;
+                if (!keep_waiting(it, t0)) {
+                    gave_up = true;
+                    break;
+                }
+                if (need && !ok) {
+                    if (!sok) st = ld_acq(q);
+                    load_x(q + 1, xa);
+                    load_x(q + 1 + NX, xp);
+                }
+            }
+            if (!(lane < stop || (p2 && lane == stop))) v = zero;
+            excl = wave_combine_ordered(v, lane, bits);
+            done = p2 != 0 || gave_up;
+            if (below == 63 && !done && lane == 0) {
+                // group aggregate of group g: tiles 64g .. 64g+63
+                const Agg ga = agg_combine(excl, own, bits);
+                if (NX) put_x(group(g) + 1, 1, ga);
+                st_rel(group(g), lb_status(1, epoch, ga.c));
+            }
+        }
+        // stage 2: whole groups, 64 per step (lane j -> group gb-1-j)
+        for (uint64_t gb = g; !done;) {
+            const int64_t h = (int64_t)gb - 1 - (int64_t)lane;
+            const bool act = h >= 0;
+            const uint64_t* lt = tile(act ? (uint64_t)h * 64 + 63 : 0);
+            const uint64_t* gg = group(act ? (uint64_t)h : 0);
+            uint64_t ls = 0, gs = 0, xl[NXA], xg[NXA];
+            if (act) {
+                ls = ld_acq(lt);
+                load_x(lt + 1 + NX, xl);
+                gs = ld_acq(gg);
+                load_x(gg + 1, xg);
+            }
+            uint32_t it = 0, stop = 64;
+            uint64_t t0 = 0, p2 = 0;
+            Agg vp = zero, vg = zero;
+            for (;;) {
+                const bool hasP = act && lb_status_ok(ls, epoch) && (ls >> 62) == 2 &&
+                                  decode(xl, 2, ls & CNT_MASK, vp);
+                const bool hasG = act && lb_status_ok(gs, epoch) && decode(xg, 1, gs & CNT_MASK, vg);
+                p2 = ballot(!act || hasP);
+                stop = p2 ? (uint32_t)__builtin_ctzll(p2) : 64;
+                const bool need = lane < stop && !hasG;
+                if (!ballot(need)) break;
+                if (!keep_waiting(it, t0)) {
+                    gave_up = true;
+                    break;
+                }
+                if (need) {
+                    ls = ld_acq(lt);
+                    load_x(lt + 1 + NX, xl);
+                    gs = ld_acq(gg);
+                    load_x(gg + 1, xg);
+                }
+            }
+            const Agg v = lane < stop ? vg : ((lane == stop && act) ? vp : zero);
+            const Agg part = wave_combine_ordered(v, lane, bits);
+            excl = agg_combine(part, excl, bits);
+            done = p2 != 0 || gave_up;
+            gb = gb >= 64 ? gb - 64 : 0;
+        }
+        if (lane == 0 && !gave_up) {
+            const Agg incl = agg_combine(excl, own, bits);
+            if (NX) put_x(D + 1 + NX, 2, incl);
+            st_rel(&D[0], lb_status(2, epoch, incl.c));
+        }
+        return excl;
+    }
+
+    // zero one slice of the tile and group descriptors not used by this
+    // launch (tiles >= T, groups >= T/64), spread over the launch's threads
+    __device__ void clear_slice(uint64_t T, uint64_t tid, uint64_t nthreads) const
+    {
+        const uint64_t per = (cap + LB_CLEAR_SLICES - 1) / LB_CLEAR_SLICES;
+        uint64_t lo = (uint64_t)(epoch % LB_CLEAR_SLICES) * per;
+        const uint64_t hi = lo + per < cap ? lo + per : cap;
+        const uint64_t glo = lo / 64 < T / 64 ? T / 64 : lo / 64, ghi = (hi + 63) / 64;
+        if (lo < T) lo = T;
+        for (uint64_t i = lo + tid; i < hi; i += nthreads)
+            for (uint32_t k = 0; k < TS; k++) st_rel(&tiles[i * TS + k], 0);
+        for (uint64_t i = glo + tid; i < ghi && i < cap / 64 + 2; i += nthreads)
+            for (uint32_t k = 0; k < GS; k++) st_rel(&groups[i * GS + k], 0);
+    }
+};
 
 } // namespace am

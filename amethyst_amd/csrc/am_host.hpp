@@ -7,6 +7,9 @@
 #include "../../include/amethyst.h"
 #include "am_device.hpp"
 
+// descriptor bytes of the largest compaction launch over n elements (k_compress.hip)
+uint64_t am_compress_desc_bytes(uint64_t n);
+
 namespace am {
 
 inline uint32_t tight_bytes(uint32_t bits)

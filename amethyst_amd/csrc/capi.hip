@@ -1,5 +1,7 @@
 // capi.hip -- helper entry points of the C ABI (sizes, width rules,
 // workspace set-up). Operators live with their kernels.
+#include <cstddef>
+
 #include "am_device.hpp"
 #include "am_host.hpp"
 
@@ -26,6 +28,7 @@ extern "C" const char* am_status_string(int status)
         case AM_E_HIP: return "HIP runtime error";
         case AM_E_WORKSPACE: return "workspace missing or too small";
         case AM_E_CAPACITY: return "output capacity too small";
+        case AM_E_ABORTED: return "look-back wait timed out; workspace poisoned until re-initialised";
     }
     return "unknown status";
 }
@@ -79,10 +82,18 @@ extern "C" uint64_t am_gen_bitmask(uint32_t bits) { return bits == 0 ? 0 : lowma
 
 extern "C" uint64_t am_workspace_bytes(uint64_t max_elements)
 {
-    // fixed part (sum partials, group-by tables) + one descriptor set per
-    // 4096-element compaction tile
-    uint64_t tiles = (max_elements + 4095) / 4096 + 1;
-    return WS_FIXED + tiles * WS_DESC_BYTES_PER_TILE;
+    // fixed part (sum partials, group-by tables) + the compaction descriptors
+    return WS_FIXED + am_compress_desc_bytes(max_elements) + 64;
+}
+
+extern "C" int am_workspace_status(const void* ws)
+{
+    if (!ws) return AM_E_INVALID;
+    uint32_t abort = 0;
+    if (hipMemcpy(&abort, (const uint8_t*)ws + offsetof(WsHeader, abort), sizeof(abort), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return AM_E_HIP;
+    return abort ? AM_E_ABORTED : AM_OK;
 }
 
 extern "C" int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream)

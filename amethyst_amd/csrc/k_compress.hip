@@ -7,26 +7,27 @@
 // re-reads an input word from global memory for every output element, and
 // merges the slab words shared by two warps with atomicAnd/atomicOr.
 //
-// Here it is one launch and it uses no contended global atomics. A grid of
-// at most as many 256-thread blocks as can be resident at once walks the
-// 4096-element tiles in order (block b takes tiles b, b+G, ...; a tile only
-// waits on lower tiles, whose blocks are running, so look-back always makes
-// progress). Each block loads the next tile while it works on the current
-// one, and per tile:
-//   1. decodes element base + 64j + lane per lane; the selection flag (mask
-//      bit or the predicate evaluated in place) goes through one wave64
-//      ballot, which is also the output mask word and, via v_mbcnt, each
-//      lane's rank among the wave's selected elements;
-//   2. publishes its aggregate and walks back over predecessors 64 at a
-//      time (one descriptor per lane), combining (count, window, overflow)
-//      with an ordered butterfly; the window is the last 63 bits of the
-//      packed output stream, so the tile that completes an output word can
-//      build it from its predecessors' values without atomics;
-//   3. writes its selected values: byte-aligned outputs (u8/u16/u32/u64
-//      containers) are scattered directly at their rank; slab and bitstream
-//      outputs are assembled one whole word per lane from the LDS-compacted
-//      tile plus the carried window, so every word is written exactly once,
-//      dead and padding bits zero, no read-modify-write.
+// Here it is one launch without contended global atomics: one 256-thread
+// block per tile, in tile order, each
+//   1. selecting: the word kernel (slab and 1/2/4/8-byte container inputs)
+//      gives every lane whole 64-bit input words, decodes all fields of a
+//      word in registers and forms the word's selection bits (predicate
+//      evaluated in place, or the bits of the input mask); per 64-word chunk
+//      a wave scan gives each lane its rank, and the chunk's eps output mask
+//      words are gathered from the lanes' bit groups by shuffles. The element
+//      kernel (bitstream inputs) decodes element base + 64j + lane per lane
+//      and uses wave64 ballots, which are the mask words themselves;
+//   2. staging its selected values, compacted, in LDS at the narrowest
+//      container that holds them;
+//   3. publishing its aggregate and looking back (one wave) over its
+//      predecessors, two-level: group-mates, then whole 64-tile groups. The
+//      scan element carries (count, window[, overflow]); the window is the
+//      last 63 bits of the packed output stream, so the tile that completes
+//      an output word builds it from its predecessors' values;
+//   4. writing: containers are copied out contiguously at the tile's offset;
+//      slab and bitstream outputs are assembled one whole word per lane, so
+//      every word is written exactly once, dead and padding bits zero, with
+//      no read-modify-write.
 // Descriptor words carry the launch epoch (bumped by the last tile), so the
 // workspace needs no clearing between launches or graph replays.
 #include "am_device.hpp"
@@ -34,22 +35,33 @@
 
 namespace am {
 
-constexpr uint32_t CS_WAVES = 4;
-constexpr uint32_t CS_J = 16;
-constexpr uint32_t CS_TILE = CS_WAVES * CS_J * 64; // 4096
+constexpr uint32_t CS_THREADS = 256;          // 4 waves
+constexpr uint32_t CW_RMAX = 4;               // word kernel: 64-word chunks per wave (max)
+constexpr uint32_t CW_LDS = 16384;            // word kernel: bytes of staged values
+constexpr uint32_t CE_J = 16;                 // element kernel: elements per lane
+constexpr uint32_t CE_TILE = 4 * CE_J * 64;   // element kernel: 4096 elements per tile
+
+template <int VB> struct VT;
+template <> struct VT<1> { using T = uint8_t; };
+template <> struct VT<2> { using T = uint16_t; };
+template <> struct VT<4> { using T = uint32_t; };
+template <> struct VT<8> { using T = uint64_t; };
 
 struct CompressArgs {
     Col in;
     const uint64_t* mask_in;
     Pred pr;
-    uint32_t chk_ovf; // output narrower than input: count values that do not fit
-    uint32_t T;       // tiles
+    uint32_t fw, eps, R; // word kernel: field width, fields per word, chunks per wave
+    uint32_t T;          // tiles
+    uint64_t in_words;   // word kernel: words holding the n elements
     Col out;
     uint64_t cap; // output capacity in elements
     uint64_t* mask_out;
     uint64_t* stats;
     WsHeader* hdr;
-    Lookback lb;
+    uint64_t* lb_tiles;
+    uint64_t* lb_groups;
+    uint64_t lb_cap;
 };
 
 __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
@@ -62,285 +74,474 @@ __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
     }
 }
 
-template <int IK, int OK, bool PRED, bool WIDE>
-__global__ __launch_bounds__(256) void k_compress(CompressArgs a)
+__device__ inline uint64_t shfl64(uint64_t v, uint32_t src)
 {
-    __shared__ uint32_t s_wcnt[CS_WAVES];
-    __shared__ uint32_t s_wovf[CS_WAVES];
-    __shared__ uint64_t s_P, s_carry;
-    __shared__ uint32_t s_epoch;
-    __shared__ uint64_t s_vals[OK == K_STRIDED ? 1 : CS_TILE];
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
+}
 
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+__device__ inline uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
+{
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)v, d);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ inline uint32_t wave_sum32(uint32_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m);
+    return v;
+}
+
+__device__ inline uint64_t readlane64(uint64_t v, uint32_t j)
+{
+    return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), j) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, j);
+}
+
+// Lane k < eps returns output mask word k of a 64-word chunk whose lane L
+// holds the selection bits of elements L*eps .. L*eps+eps-1 in g.
+__device__ inline uint64_t mask_gather(uint64_t g, uint32_t eps, uint32_t lane)
+{
+    if (eps == 1) return ballot(g & 1);
+    const uint32_t k = lane < eps ? lane : 0;
+    const uint32_t L0 = (64 * k) / eps;
+    const uint32_t nl = 63 / eps + 2;
+    uint64_t mw = 0;
+    for (uint32_t i = 0; i < nl; i++) {
+        const uint32_t L = L0 + i;
+        const uint64_t gl = eps <= 32 ? (uint64_t)(uint32_t)__shfl((int)(uint32_t)g, (int)(L & 63))
+                                      : shfl64(g, L & 63);
+        const int32_t off = (int32_t)(L * eps) - (int32_t)(64 * k);
+        if (L < 64 && off < 64) mw |= off >= 0 ? gl << off : gl >> (-off);
+    }
+    return mw;
+}
+
+// Window of a tile: the last 63 bits of its packed output stream.
+template <typename V>
+__device__ inline uint64_t tile_window(const V* s_vals, uint32_t ctile, uint32_t bo, uint32_t lane)
+{
+    uint64_t w = 0;
+    if (bo < 64) {
+        const uint32_t k = (63 + bo - 1) / bo;
+        const uint32_t take = ctile < k ? ctile : k;
+        if (lane < take) {
+            const uint32_t jx = ctile - take + lane;
+            const int64_t shift = (int64_t)jx * bo - ((int64_t)ctile * bo - 63);
+            const uint64_t v = s_vals[jx];
+            w = shift >= 0 ? (v << shift) : (v >> (-shift));
+        }
+        w = wave_or_u64(w) & WIN_BITS;
+    }
+    return w;
+}
+
+// Look-back for tile t (one wave); the last tile writes the statistics and
+// advances the epoch.
+template <uint32_t F>
+__device__ inline void tile_lookback(const CompressArgs& a, const Lookback<F>& L, uint64_t t, uint32_t ctile,
+                                     uint32_t otile, uint64_t own_w, uint32_t lane, uint64_t* s_P, uint64_t* s_carry)
+{
+    const Agg own{ctile, own_w, otile, 0};
+    const Agg ex = L.run(t, own, lane);
+    if (lane == 0) {
+        *s_P = ex.c;
+        *s_carry = ex.w;
+        if (t == a.T - 1) {
+            const bool bad = L.aborted();
+            if (a.stats) {
+                a.stats[0] = bad ? STATS_INVALID : ex.c + ctile;
+                a.stats[1] = ovf_add(ex.o, otile);
+            }
+            st32(&a.hdr->epoch, L.epoch + 1);
+        }
+    }
+}
+
+// Write the tile's ctile staged values at output position P.
+template <int OK, typename V>
+__device__ inline void tile_write(const CompressArgs& a, const V* s_vals, uint64_t P, uint64_t carry,
+                                  uint32_t ctile, bool last, uint32_t tid)
+{
     const uint32_t bo = a.out.bits;
     const uint64_t om = a.out.mask;
-    const uint64_t n = a.in.n;
-    const uint32_t T = a.T;
-
-    if (tid == 0) s_epoch = ld32(&a.hdr->epoch);
-    __syncthreads();
-    const uint32_t epoch = s_epoch;
-    {
-        // zero one slice of the descriptors this launch does not use
-        const uint64_t cap = a.lb.cap;
-        const uint64_t per = (cap + LB_CLEAR_SLICES - 1) / LB_CLEAR_SLICES;
-        uint64_t lo = (uint64_t)(epoch % LB_CLEAR_SLICES) * per;
-        uint64_t hi = lo + per < cap ? lo + per : cap;
-        if (lo < T) lo = T;
-        for (uint64_t i = lo + (uint64_t)blockIdx.x * 256 + tid; i < hi; i += (uint64_t)gridDim.x * 256) {
-            st_rel(&a.lb.status[i], 0);
-            st_rel(&a.lb.winA[2 * i], 0);
-            st_rel(&a.lb.winA[2 * i + 1], 0);
-            st_rel(&a.lb.winP[2 * i], 0);
-            st_rel(&a.lb.winP[2 * i + 1], 0);
+    const uint64_t ocap = a.cap;
+    if (OK == K_STRIDED) {
+        for (uint32_t i = tid; i < ctile; i += CS_THREADS) {
+            const uint64_t r = P + i;
+            if (r < ocap) store_container(a.out, r, s_vals[i]);
+        }
+        if (last && tid == 0) {
+            uint64_t K = P + ctile;
+            if (K > ocap) K = ocap;
+            const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
+            for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
         }
     }
-    uint32_t t = blockIdx.x;
-    if (t >= T) return;
+    else if (OK == K_SLAB) {
+        const uint64_t eps = a.out.eps;
+        const uint64_t kf = P / eps;
+        uint64_t kl = (P + ctile) / eps;
+        if (last && ((P + ctile) % eps)) kl += 1;
+        const uint64_t wcap = (ocap + eps - 1) / eps;
+        if (kl > wcap) kl = wcap;
+        for (uint64_t k = kf + tid; k < kl; k += CS_THREADS) {
+            uint64_t word = 0;
+            const uint64_t e0 = k * eps;
+            for (uint32_t s = 0; s < eps; s++) {
+                const uint64_t e = e0 + s;
+                if (e >= ocap) break;
+                uint64_t v;
+                if (e < P) v = (carry >> (63 - (P - e) * bo)) & om;
+                else if (e < P + ctile) v = s_vals[e - P];
+                else break;
+                word |= v << (s * bo);
+            }
+            a.out.p[k] = word;
+        }
+    }
+    else {
+        const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
+        const uint64_t kf = B0 >> 6;
+        uint64_t kl = B1 >> 6;
+        if (last && (B1 & 63)) kl += 1;
+        const uint64_t wcap = (Bcap + 63) >> 6;
+        if (kl > wcap) kl = wcap;
+        for (uint64_t k = kf + tid; k < kl; k += CS_THREADS) {
+            const uint64_t b0 = k * 64;
+            uint64_t word = 0;
+            if (b0 < B0) word = carry >> (b0 + 63 - B0);
+            uint64_t e0 = b0 / bo;
+            if (e0 < P) e0 = P;
+            uint64_t e1 = (b0 + 63) / bo + 1;
+            if (e1 > P + ctile) e1 = P + ctile;
+            for (uint64_t e = e0; e < e1; e++) {
+                const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
+                const uint64_t v = s_vals[e - P];
+                word |= pos >= 0 ? (v << pos) : (v >> (-pos));
+            }
+            if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
+            a.out.p[k] = word;
+        }
+    }
+}
 
-    Fetch<IK, WIDE> f[CS_J];
-    uint64_t mk = 0; // lane j < CS_J: mask word j of this wave's chunk (mask mode)
-    auto fetch_tile = [&](uint32_t tt, Fetch<IK, WIDE>* ff, uint64_t& mm) {
-        const uint64_t wb = (uint64_t)tt * CS_TILE + wave * (CS_J * 64);
-        if (wb < n) {
-            const uint32_t rem = (uint32_t)(n - wb < CS_J * 64 ? n - wb : CS_J * 64);
-            TileReader<IK> rd(a.in, wb);
+template <uint32_t F>
+__device__ inline Lookback<F> make_lookback(const CompressArgs& a, uint32_t epoch)
+{
+    Lookback<F> L{};
+    L.tiles = a.lb_tiles;
+    L.groups = a.lb_groups;
+    L.cap = a.lb_cap;
+    L.epoch = epoch;
+    L.bits = a.out.bits;
+    L.hdr = a.hdr;
+    return L;
+}
+
+template <int OK, bool OVF>
+struct CsFlags {
+    static constexpr uint32_t F = OK != K_STRIDED ? (LB_WIN | LB_OVF) : (OVF ? LB_OVF : 0);
+};
+
+// ------------------------------------------------------------ word kernel
+//
+// Tile = 4 waves x R chunks x 64 words (R = 4, 2, 1 for eps <= 8, <= 16,
+// more), i.e. 256*R*eps elements, a multiple of 64, so every tile starts on
+// an output mask word.
+template <int OK, bool PRED, bool OVF, int VB, bool WIDE>
+__global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
+{
+    constexpr uint32_t F = CsFlags<OK, OVF>::F;
+    using V = typename VT<VB>::T;
+    __shared__ V s_vals[CW_LDS / VB];
+    __shared__ uint32_t s_ccnt[4 * CW_RMAX], s_covf[4 * CW_RMAX];
+    __shared__ uint64_t s_P, s_carry;
+    __shared__ uint32_t s_epoch, s_abort;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t t = blockIdx.x;
+    if (tid == 0) {
+        s_epoch = ld32(&a.hdr->epoch);
+        s_abort = ld32(&a.hdr->abort);
+    }
+    __syncthreads();
+    if (s_abort) {
+        if (t == a.T - 1 && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
+        return;
+    }
+    const Lookback<F> L = make_lookback<F>(a, s_epoch);
+    L.clear_slice(a.T, t * CS_THREADS + tid, (uint64_t)a.T * CS_THREADS);
+
+    const uint32_t fw = a.fw, eps = a.eps, R = a.R;
+    const uint64_t n = a.in.n;
+    const uint64_t vm = a.in.mask & a.out.mask; // staged value
+    const uint64_t om = a.out.mask;
+    const uint64_t TW = (uint64_t)CS_THREADS * R; // words per tile
+    const uint64_t nmw = (n + 63) >> 6;
+
+    uint64_t x[CW_RMAX], g[CW_RMAX], mk[CW_RMAX];
+    uint32_t lp[CW_RMAX];
 #pragma unroll
-            for (uint32_t j = 0; j < CS_J; j++) {
-                const uint32_t i = j * 64 + lane;
-                ff[j].load(rd, i < rem ? i : rem - 1);
+    for (uint32_t r = 0; r < CW_RMAX; r++) {
+        x[r] = 0;
+        mk[r] = 0;
+        if (r < R) {
+            const uint64_t wi = t * TW + (uint64_t)(wave * R + r) * 64 + lane;
+            if (wi < a.in_words) x[r] = a.in.p[wi];
+            if (!PRED && lane < eps) {
+                const uint64_t mw = ((t * TW + (uint64_t)(wave * R + r) * 64) * eps >> 6) + lane;
+                if (mw < nmw) mk[r] = a.mask_in[mw];
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < CW_RMAX; r++) {
+        g[r] = 0;
+        lp[r] = 0;
+        if (r >= R) continue;
+        const uint64_t w0 = t * TW + (uint64_t)(wave * R + r) * 64; // first word of the chunk
+        const uint64_t e0 = (w0 + lane) * eps;
+        const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
+        uint64_t gg = 0, go = 0;
+        if (PRED) {
+            for (uint32_t s = 0; s < eps; s++) {
+                const uint64_t v = (x[r] >> (s * fw)) & a.in.mask;
+                const bool sel = WIDE ? a.pr(v) : a.pr.sel32((uint32_t)v);
+                gg |= (uint64_t)sel << s;
+                if (OVF) go |= (uint64_t)(sel && (v & ~om) != 0) << s;
             }
         }
         else {
-#pragma unroll
-            for (uint32_t j = 0; j < CS_J; j++) ff[j].raw = 0, ff[j].sh = 0;
+            const uint32_t pos = lane * eps, k1 = pos >> 6, off = pos & 63;
+            const uint64_t lo = shfl64(mk[r], k1);
+            const uint64_t hi = shfl64(mk[r], (k1 + 1) & 63);
+            gg = (lo >> off) | ((off && off + eps > 64) ? hi << (64 - off) : 0);
+            if (OVF)
+                for (uint32_t s = 0; s < eps; s++) {
+                    const uint64_t v = (x[r] >> (s * fw)) & a.in.mask;
+                    go |= (uint64_t)((v & ~om) != 0) << s;
+                }
         }
-        if (!PRED) {
-            const uint64_t w = (wb >> 6) + lane;
-            mm = (lane < CS_J && w < ((n + 63) >> 6)) ? a.mask_in[w] : 0;
-        }
-    };
-    fetch_tile(t, f, mk);
-
-    for (; t < T; t += gridDim.x) {
-        // ---- prefetch the next tile of this block
-        Fetch<IK, WIDE> nf[CS_J];
-        uint64_t nmk = 0;
-        const uint32_t tn = t + gridDim.x;
-        if (tn < T) fetch_tile(tn, nf, nmk);
-
-        // ---- 1. decode, select, count (lane j keeps ballot j in `mine`)
-        const uint64_t wbase = (uint64_t)t * CS_TILE + wave * (CS_J * 64);
-        const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < CS_J * 64 ? n - wbase : CS_J * 64) : 0;
-        uint64_t mine = 0;
-        uint32_t cnt = 0, ovf = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < CS_J; j++) {
-            const bool inb = j * 64 + lane < wrem;
-            const uint64_t v = f[j].value(a.in.mask);
-            bool fl;
-            if (PRED) {
-                fl = inb && (WIDE ? a.pr(v) : a.pr.sel32((uint32_t)v));
-            }
-            else {
-                const uint64_t mw = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mk >> 32), j) << 32) |
-                                    (uint32_t)__builtin_amdgcn_readlane((uint32_t)mk, j);
-                fl = inb && ((mw >> lane) & 1);
-            }
-            const uint64_t b = ballot(fl);
-            if (lane == j) mine = b;
-            cnt += (uint32_t)__popcll(b);
-            if (a.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && (v & ~om) != 0));
+        gg &= lowmask(nv);
+        g[r] = gg;
+        const uint32_t c = (uint32_t)__popcll(gg);
+        const uint32_t inc = wave_incl_scan(c, lane);
+        lp[r] = inc - c;
+        if (lane == 63) s_ccnt[wave * R + r] = inc;
+        if (OVF) {
+            const uint32_t oc = wave_sum32((uint32_t)__popcll(go & gg));
+            if (lane == 0) s_covf[wave * R + r] = oc;
         }
         if (a.mask_out) {
-            const uint64_t w = (wbase >> 6) + lane;
-            if (lane < CS_J && w < ((n + 63) >> 6)) a.mask_out[w] = mine;
+            const uint64_t mw = mask_gather(gg, eps, lane);
+            const uint64_t mi = ((w0 * eps) >> 6) + lane;
+            if (lane < eps && mi < nmw) a.mask_out[mi] = mw;
         }
-        if (lane == 0) {
-            s_wcnt[wave] = cnt;
-            s_wovf[wave] = ovf;
-        }
-        __syncthreads();
-        uint32_t wpre = 0, ctile = 0, otile = 0;
+    }
+    __syncthreads();
+    const uint32_t NC = 4 * R;
+    const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
+    const uint32_t cinc = wave_incl_scan(cc, lane);
+    const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
+    uint32_t otile = 0;
+    if (OVF) otile = wave_sum32(lane < NC ? s_covf[lane] : 0);
+    if (OK == K_STRIDED && wave == 0) tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, &s_carry);
+    // stage the selected values, compacted
 #pragma unroll
-        for (uint32_t w = 0; w < CS_WAVES; w++) {
-            const uint32_t c = s_wcnt[w];
-            wpre += w < wave ? c : 0;
-            ctile += c;
-            otile += s_wovf[w];
-        }
-        auto bal = [&](uint32_t j) -> uint64_t {
-            return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
-                   (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
-        };
-
-        if (OK != K_STRIDED) {
-            uint32_t run = wpre;
-#pragma unroll
-            for (uint32_t j = 0; j < CS_J; j++) {
-                const uint64_t b = bal(j);
-                if ((b >> lane) & 1) s_vals[run + mbcnt(b)] = f[j].value(a.in.mask) & om;
-                run += (uint32_t)__popcll(b);
-            }
-            __syncthreads();
-        }
-
-        // ---- 2. look-back (wave 0)
+    for (uint32_t r = 0; r < CW_RMAX; r++) {
+        if (r >= R) continue;
+        const uint32_t ci = wave * R + r;
+        uint32_t pos = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r];
+        const uint64_t gg = g[r], xr = x[r];
+        for (uint32_t s = 0; s < eps; s++)
+            if ((gg >> s) & 1) s_vals[pos++] = (V)((xr >> (s * fw)) & vm);
+    }
+    __syncthreads();
+    if (OK != K_STRIDED) {
         if (wave == 0) {
-            uint64_t own_w = 0;
-            if (OK != K_STRIDED && bo < 64) {
-                const uint32_t k = (63 + bo - 1) / bo;
-                const uint32_t take = ctile < k ? ctile : k;
-                if (lane < take) {
-                    const uint32_t jx = ctile - take + lane;
-                    const int64_t shift = (int64_t)jx * bo - ((int64_t)ctile * bo - 63);
-                    const uint64_t val = s_vals[jx];
-                    own_w = shift >= 0 ? (val << shift) : (val >> (-shift));
-                }
-                own_w = wave_or_u64(own_w) & WIN_BITS;
-            }
-            uint64_t excl_c = 0, excl_w = 0, excl_o = 0;
-            if (t > 0) {
-                if (lane == 0) {
-                    st_rel(&a.lb.winA[2 * t], lb_word(1, epoch, own_w));
-                    st_rel(&a.lb.winA[2 * t + 1], lb_word(1, epoch, (own_w >> 46) | ((uint64_t)otile << 17)));
-                    st_rel(&a.lb.status[t], lb_status(1, epoch, ctile));
-                }
-                int64_t pos = (int64_t)t - 1;
-                while (true) {
-                    const int64_t idx = pos - (int64_t)lane;
-                    uint64_t s = 0;
-                    uint32_t lvl = 2;
-                    if (idx >= 0) {
-                        while (!lb_status_ok(s = ld_acq(&a.lb.status[idx]), epoch)) __builtin_amdgcn_s_sleep(1);
-                        lvl = (uint32_t)(s >> 62);
-                    }
-                    const uint64_t pm = ballot(lvl == 2);
-                    const uint32_t stop = pm ? (uint32_t)__builtin_ctzll(pm) : 64;
-                    const bool incl = lane <= stop && idx >= 0;
-                    uint64_t c = 0, w = 0, o = 0;
-                    if (incl) {
-                        c = s & CNT_MASK;
-                        const uint64_t* wp = (lvl == 2 ? a.lb.winP : a.lb.winA) + 2 * idx;
-                        uint64_t w0, w1;
-                        while (!lb_word_ok(w0 = ld_acq(wp), lvl, epoch)) __builtin_amdgcn_s_sleep(1);
-                        while (!lb_word_ok(w1 = ld_acq(wp + 1), lvl, epoch)) __builtin_amdgcn_s_sleep(1);
-                        w = (w0 & P46) | ((w1 & 0x1FFFF) << 46);
-                        o = (w1 & P46) >> 17;
-                    }
-                    // ordered butterfly: higher lanes hold older tiles
-                    for (int m = 1; m < 64; m <<= 1) {
-                        const uint64_t oc = shfl_xor_u64(c, m), ow = shfl_xor_u64(w, m), oo = shfl_xor_u64(o, m);
-                        if (lane & m) w = win_combine(w, ow, oc, bo);
-                        else w = win_combine(ow, w, c, bo);
-                        c += oc;
-                        o = ovf_add(o, oo);
-                    }
-                    excl_w = win_combine(w, excl_w, excl_c, bo);
-                    excl_c += c;
-                    excl_o = ovf_add(excl_o, o);
-                    if (pm) break;
-                    pos -= 64;
-                }
-            }
-            const uint64_t incl_c = excl_c + ctile;
-            const uint64_t incl_w = win_combine(excl_w, own_w, ctile, bo);
-            const uint64_t incl_o = ovf_add(excl_o, otile);
-            if (lane == 0) {
-                st_rel(&a.lb.winP[2 * t], lb_word(2, epoch, incl_w));
-                st_rel(&a.lb.winP[2 * t + 1], lb_word(2, epoch, (incl_w >> 46) | (incl_o << 17)));
-                st_rel(&a.lb.status[t], lb_status(2, epoch, incl_c));
-                s_P = excl_c;
-                s_carry = excl_w;
-                if (t == T - 1) {
-                    if (a.stats) {
-                        a.stats[0] = incl_c;
-                        a.stats[1] = incl_o;
-                    }
-                    st32(&a.hdr->epoch, epoch + 1);
-                }
-            }
+            const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
+            tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
         }
         __syncthreads();
-        const uint64_t P = s_P, carry = s_carry;
-        const bool last = (t == T - 1);
-        const uint64_t ocap = a.cap;
+    }
+    tile_write<OK>(a, s_vals, s_P, s_carry, ctile, t == a.T - 1, tid);
+}
 
-        // ---- 3. write
-        if (OK == K_STRIDED) {
-            uint32_t run = wpre;
+// ------------------------------------------------------------ element kernel
+//
+// Bitstream inputs: 4096-element tiles, element base + 64j + lane per lane.
+template <int OK, bool PRED, bool OVF, bool WIDE>
+__global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
+{
+    constexpr uint32_t F = CsFlags<OK, OVF>::F;
+    constexpr int IK = K_BITSTREAM;
+    __shared__ uint64_t s_vals[CE_TILE];
+    __shared__ uint32_t s_wcnt[4], s_wovf[4];
+    __shared__ uint64_t s_P, s_carry;
+    __shared__ uint32_t s_epoch, s_abort;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t t = blockIdx.x;
+    if (tid == 0) {
+        s_epoch = ld32(&a.hdr->epoch);
+        s_abort = ld32(&a.hdr->abort);
+    }
+    __syncthreads();
+    if (s_abort) {
+        if (t == a.T - 1 && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
+        return;
+    }
+    const Lookback<F> L = make_lookback<F>(a, s_epoch);
+    L.clear_slice(a.T, t * CS_THREADS + tid, (uint64_t)a.T * CS_THREADS);
+
+    const uint64_t n = a.in.n;
+    const uint64_t om = a.out.mask;
+    const uint64_t nmw = (n + 63) >> 6;
+    const uint64_t wb = t * CE_TILE + (uint64_t)wave * (CE_J * 64);
+    const uint32_t wrem = wb < n ? (uint32_t)(n - wb < CE_J * 64 ? n - wb : CE_J * 64) : 0;
+    const TileReader<IK> rd(a.in, wb < n ? wb : 0);
+    auto idx = [&](uint32_t j) { return j * 64 + lane < wrem ? j * 64 + lane : (wrem ? wrem - 1 : 0); };
+    Fetch<IK, WIDE> f[CE_J];
+    uint64_t mk = 0;
+    if (wrem) {
 #pragma unroll
-            for (uint32_t j = 0; j < CS_J; j++) {
-                const uint64_t b = bal(j);
-                if ((b >> lane) & 1) {
-                    const uint64_t r = P + run + mbcnt(b);
-                    if (r < ocap) store_container(a.out, r, f[j].value(a.in.mask));
-                }
-                run += (uint32_t)__popcll(b);
-            }
-            if (last && tid == 0) {
-                uint64_t K = P + ctile;
-                if (K > ocap) K = ocap;
-                const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
-                for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
-            }
+        for (uint32_t j = 0; j < CE_J; j++) f[j].load(rd, idx(j));
+        if (!PRED) {
+            const uint64_t w = (wb >> 6) + lane;
+            mk = (lane < CE_J && w < nmw) ? a.mask_in[w] : 0;
         }
-        else if (OK == K_SLAB) {
-            const uint64_t eps = a.out.eps;
-            const uint64_t kf = P / eps;
-            uint64_t kl = (P + ctile) / eps;
-            if (last && ((P + ctile) % eps)) kl += 1;
-            const uint64_t wcap = (ocap + eps - 1) / eps;
-            if (kl > wcap) kl = wcap;
-            for (uint64_t k = kf + tid; k < kl; k += 256) {
-                uint64_t word = 0;
-                const uint64_t e0 = k * eps;
-                for (uint32_t s = 0; s < eps; s++) {
-                    const uint64_t e = e0 + s;
-                    if (e >= ocap) break;
-                    uint64_t val;
-                    if (e < P) val = (carry >> (63 - (P - e) * bo)) & om;
-                    else if (e < P + ctile) val = s_vals[e - P];
-                    else break;
-                    word |= val << (s * bo);
-                }
-                a.out.p[k] = word;
-            }
+    }
+    uint64_t mine = 0;
+    uint32_t cnt = 0, ovf = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < CE_J; j++) {
+        const bool inb = j * 64 + lane < wrem;
+        const uint64_t v = wrem ? f[j].value(rd, idx(j)) : 0;
+        bool fl;
+        if (PRED) fl = inb && (WIDE ? a.pr(v) : a.pr.sel32((uint32_t)v));
+        else fl = inb && ((readlane64(mk, j) >> lane) & 1);
+        const uint64_t b = ballot(fl);
+        if (lane == j) mine = b;
+        cnt += (uint32_t)__popcll(b);
+        if (OVF) ovf += (uint32_t)__popcll(ballot(fl && (v & ~om) != 0));
+    }
+    if (a.mask_out) {
+        const uint64_t w = (wb >> 6) + lane;
+        if (lane < CE_J && w < nmw) a.mask_out[w] = mine;
+    }
+    if (lane == 0) {
+        s_wcnt[wave] = cnt;
+        s_wovf[wave] = ovf;
+    }
+    __syncthreads();
+    uint32_t wpre = 0, ctile = 0, otile = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++) {
+        const uint32_t c = s_wcnt[w];
+        wpre += w < wave ? c : 0;
+        ctile += c;
+        otile += s_wovf[w];
+    }
+    if (OK == K_STRIDED && wave == 0) tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, &s_carry);
+    uint32_t run = wpre;
+#pragma unroll
+    for (uint32_t j = 0; j < CE_J; j++) {
+        const uint64_t b = readlane64(mine, j);
+        if ((b >> lane) & 1) s_vals[run + mbcnt(b)] = f[j].value(rd, idx(j)) & om;
+        run += (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (OK != K_STRIDED) {
+        if (wave == 0) {
+            const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
+            tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
+        }
+        __syncthreads();
+    }
+    tile_write<OK>(a, s_vals, s_P, s_carry, ctile, t == a.T - 1, tid);
+}
+
+// ------------------------------------------------------------ host dispatch
+
+template <int OK, bool PRED, bool OVF>
+static void launch_words(const CompressArgs& a, uint32_t vb, bool wide, hipStream_t s)
+{
+    const dim3 grid(a.T), block(CS_THREADS);
+    if (wide) {
+        switch (vb) {
+            case 1: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 1, true>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 2, true>), grid, block, 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 4, true>), grid, block, 0, s, a); break;
+            default: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 8, true>), grid, block, 0, s, a);
+        }
+    }
+    else {
+        switch (vb) {
+            case 1: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 1, false>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 2, false>), grid, block, 0, s, a); break;
+            default: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 4, false>), grid, block, 0, s, a);
+        }
+    }
+}
+
+template <int OK, bool PRED, bool OVF>
+static void launch_elems(const CompressArgs& a, bool wide, hipStream_t s)
+{
+    const dim3 grid(a.T), block(CS_THREADS);
+    if (wide) hipLaunchKernelGGL((k_compress_elems<OK, PRED, OVF, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_compress_elems<OK, PRED, OVF, false>), grid, block, 0, s, a);
+}
+
+template <int OK>
+static void launch_out(const CompressArgs& a, bool words, bool pred, bool ovf, uint32_t vb, bool wide, hipStream_t s)
+{
+    if (words) {
+        if (pred) {
+            if (ovf) launch_words<OK, true, true>(a, vb, wide, s);
+            else launch_words<OK, true, false>(a, vb, wide, s);
         }
         else {
-            const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
-            const uint64_t kf = B0 >> 6;
-            uint64_t kl = B1 >> 6;
-            if (last && (B1 & 63)) kl += 1;
-            const uint64_t wcap = (Bcap + 63) >> 6;
-            if (kl > wcap) kl = wcap;
-            for (uint64_t k = kf + tid; k < kl; k += 256) {
-                const uint64_t b0 = k * 64;
-                uint64_t word = 0;
-                if (b0 < B0) word = carry >> (b0 + 63 - B0);
-                uint64_t e0 = b0 / bo;
-                if (e0 < P) e0 = P;
-                uint64_t e1 = (b0 + 63) / bo + 1;
-                if (e1 > P + ctile) e1 = P + ctile;
-                for (uint64_t e = e0; e < e1; e++) {
-                    const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
-                    const uint64_t val = s_vals[e - P];
-                    word |= pos >= 0 ? (val << pos) : (val >> (-pos));
-                }
-                if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-                a.out.p[k] = word;
-            }
+            if (ovf) launch_words<OK, false, true>(a, vb, wide, s);
+            else launch_words<OK, false, false>(a, vb, wide, s);
         }
-        __syncthreads(); // LDS reuse by the next tile
-#pragma unroll
-        for (uint32_t j = 0; j < CS_J; j++) f[j] = nf[j];
-        mk = nmk;
     }
+    else {
+        if (pred) {
+            if (ovf) launch_elems<OK, true, true>(a, wide, s);
+            else launch_elems<OK, true, false>(a, wide, s);
+        }
+        else {
+            if (ovf) launch_elems<OK, false, true>(a, wide, s);
+            else launch_elems<OK, false, false>(a, wide, s);
+        }
+    }
+}
+
+static inline uint32_t value_bytes(uint32_t bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : (bits <= 32 ? 4 : 8)); }
+
+// look-back flags of a launch (descriptor layout)
+static inline uint32_t cs_flags(uint32_t out_kind, bool ovf)
+{
+    return out_kind != K_STRIDED ? (LB_WIN | LB_OVF) : (ovf ? LB_OVF : 0);
 }
 
 } // namespace am
 
 using namespace am;
+
+// Largest descriptor footprint of any compaction launch over n elements.
+uint64_t am_compress_desc_bytes(uint64_t n)
+{
+    const uint64_t tiles = n / 1024 + 2; // tiles hold at least 1024 elements
+    return Lookback<LB_WIN | LB_OVF>::bytes_for(tiles);
+}
 
 extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const am_predicate* pred, am_column* out,
                                 uint64_t* mask_out, uint64_t* stats_dev, void* ws, uint64_t ws_bytes, void* stream)
@@ -353,50 +554,55 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     if (pred && pred->op > AM_PRED_NE) return AM_E_INVALID;
     if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
     if (out->n && !out->data) return AM_E_INVALID;
-    const uint64_t T = ci.n ? (ci.n + CS_TILE - 1) / CS_TILE : 1;
-    Lookback lb = lookback_of(ws, ws_bytes);
-    if (T > lb.cap || T >= (1ull << 31)) return AM_E_WORKSPACE;
     CompressArgs a{};
     a.in = ci;
     a.mask_in = mask;
     if (pred) a.pr = make_pred(*pred, ci.mask);
-    a.chk_ovf = co.bits < ci.bits ? 1 : 0;
-    a.T = (uint32_t)T;
     a.out = co;
     a.cap = out->n;
     a.mask_out = mask_out;
     a.stats = stats_dev;
     a.hdr = (WsHeader*)ws;
-    a.lb = lb;
-    hipStream_t s = (hipStream_t)stream;
-    const bool use_pred = (mask == nullptr);
+    const bool words = ci.kind != K_BITSTREAM;
+    uint64_t T;
+    if (words) {
+        a.fw = ci.kind == K_SLAB ? ci.bits : 8 * ci.cbytes;
+        a.eps = 64 / a.fw;
+        a.R = a.eps <= 8 ? 4 : (a.eps <= 16 ? 2 : 1);
+        a.in_words = (ci.n + a.eps - 1) / a.eps;
+        const uint64_t TW = (uint64_t)CS_THREADS * a.R;
+        T = a.in_words ? (a.in_words + TW - 1) / TW : 1;
+    }
+    else {
+        T = ci.n ? (ci.n + CE_TILE - 1) / CE_TILE : 1;
+    }
+    const bool ovf = co.bits < ci.bits;
+    const uint32_t F = cs_flags(co.kind, ovf);
+    uint8_t* base = (uint8_t*)ws + WS_FIXED;
+    const uint64_t bytes = ws_bytes - WS_FIXED;
+    uint64_t cap;
+    if (F == 0) {
+        const auto L = Lookback<0>::at(base, bytes);
+        a.lb_tiles = L.tiles, a.lb_groups = L.groups, cap = L.cap;
+    }
+    else if (F == LB_OVF) {
+        const auto L = Lookback<LB_OVF>::at(base, bytes);
+        a.lb_tiles = L.tiles, a.lb_groups = L.groups, cap = L.cap;
+    }
+    else {
+        const auto L = Lookback<LB_WIN | LB_OVF>::at(base, bytes);
+        a.lb_tiles = L.tiles, a.lb_groups = L.groups, cap = L.cap;
+    }
+    if (T > cap || T >= (1ull << 31)) return AM_E_WORKSPACE;
+    a.lb_cap = cap;
+    a.T = (uint32_t)T;
+    const uint32_t vb = value_bytes(ci.bits < co.bits ? ci.bits : co.bits);
     const bool wide = ci.bits > 32;
-#define CS_GO(IK, OK, P, W)                                                                       \
-    {                                                                                             \
-        static uint32_t cap_blocks = 0;                                                           \
-        if (!cap_blocks) cap_blocks = resident_blocks(k_compress<IK, OK, P, W>, 256, 0);          \
-        const uint32_t g = (uint32_t)(T < cap_blocks ? T : cap_blocks);                           \
-        hipLaunchKernelGGL((k_compress<IK, OK, P, W>), dim3(g), dim3(256), 0, s, a);              \
+    hipStream_t s = (hipStream_t)stream;
+    switch (co.kind) {
+        case K_STRIDED: launch_out<K_STRIDED>(a, words, pred != nullptr && !mask, ovf, vb, wide, s); break;
+        case K_SLAB: launch_out<K_SLAB>(a, words, pred != nullptr && !mask, ovf, vb, wide, s); break;
+        default: launch_out<K_BITSTREAM>(a, words, pred != nullptr && !mask, ovf, vb, wide, s); break;
     }
-#define CS_LAUNCH(IK, OK)                                    \
-    if (use_pred) {                                          \
-        if (wide) CS_GO(IK, OK, true, true) else CS_GO(IK, OK, true, false)     \
-    } else {                                                 \
-        if (wide) CS_GO(IK, OK, false, true) else CS_GO(IK, OK, false, false)   \
-    }
-#define CS_OUT(IK)                                          \
-    switch (co.kind) {                                      \
-        case K_STRIDED: CS_LAUNCH(IK, K_STRIDED); break;    \
-        case K_SLAB: CS_LAUNCH(IK, K_SLAB); break;          \
-        default: CS_LAUNCH(IK, K_BITSTREAM); break;         \
-    }
-    switch (ci.kind) {
-        case K_STRIDED: CS_OUT(K_STRIDED); break;
-        case K_SLAB: CS_OUT(K_SLAB); break;
-        default: CS_OUT(K_BITSTREAM); break;
-    }
-#undef CS_OUT
-#undef CS_LAUNCH
-#undef CS_GO
     return check_launch();
 }

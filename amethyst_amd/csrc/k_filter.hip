@@ -34,6 +34,7 @@ __global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_
     uint32_t t = blockIdx.x;
     if (t >= T) return;
     Fetch<IK, WIDE> f[FT_J];
+    auto chunk = [&](uint32_t tt) { return (uint64_t)tt * FT_TILE + wave * (FT_J * 64); };
     auto fetch_tile = [&](uint32_t tt, Fetch<IK, WIDE>* ff) {
         const uint64_t wb = (uint64_t)tt * FT_TILE + wave * (FT_J * 64);
         if (wb < n) {
@@ -50,12 +51,14 @@ __global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_
     for (; t < T; t += gridDim.x) {
         Fetch<IK, WIDE> nf[FT_J];
         if (t + gridDim.x < T) fetch_tile(t + gridDim.x, nf);
-        const uint64_t wbase = (uint64_t)t * FT_TILE + wave * (FT_J * 64);
+        const uint64_t wbase = chunk(t);
         const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < FT_J * 64 ? n - wbase : FT_J * 64) : 0;
+        const TileReader<IK> rd(in, wbase < n ? wbase : 0);
         uint64_t mine = 0;
 #pragma unroll
         for (uint32_t j = 0; j < FT_J; j++) {
-            const uint64_t v = f[j].value(in.mask);
+            const uint32_t i = j * 64 + lane < wrem ? j * 64 + lane : (wrem ? wrem - 1 : 0);
+            const uint64_t v = f[j].value(rd, i);
             const bool fl = (j * 64 + lane < wrem) && (WIDE ? pr(v) : pr.sel32((uint32_t)v));
             const uint64_t b = ballot(fl);
             if (lane == j) mine = b;

@@ -1,7 +1,7 @@
 // k_pipeline.hip -- the per-column operator chain
 //     mask = filter(a, pred); s = compressstore(a, mask) + compressstore(b, mask);
 //     total = sum(s)
-// for a whole batch of columns of uneven length and width in ONE persistent
+// for a whole batch of columns of uneven length and width in ONE
 // launch (plus a one-block epoch bump), capturable in a hipGraph.
 //
 // The reference has no fused chain: "op chaining" there means one operator's
@@ -30,7 +30,6 @@
 namespace am {
 
 constexpr uint32_t PL_WAVES = 4;
-constexpr uint32_t PL_DESC_WORDS = 9; // status + 4 words per level
 // kernel variants: J elements per lane per tile -> tiles of 4*J*64 elements
 __host__ __device__ constexpr uint32_t pl_tile(uint32_t J) { return PL_WAVES * J * 64; }
 
@@ -43,6 +42,8 @@ struct PipeJob {
     uint32_t T;       // tiles of this column (>= 1)
     uint32_t vbits;   // width of a+b (w+1, capped at 64)
     uint32_t chk_ovf; // s narrower than vbits
+    uint32_t gbase;   // first look-back group of this column
+    uint32_t pad;
 };
 
 struct PipeArgs {
@@ -50,8 +51,8 @@ struct PipeArgs {
     uint32_t njobs;
     uint32_t total_tiles;
     WsHeader* hdr;
-    uint64_t* desc; // [cap][9]
-    uint64_t cap;
+    uint64_t* lb_tiles;  // tile descriptors, indexed by global tile
+    uint64_t* lb_groups; // group descriptors, per column from its gbase
 };
 
 __device__ inline uint32_t pl_find(const PipeJob* jobs, uint32_t n, uint32_t tile)
@@ -69,240 +70,168 @@ template <int IK, uint32_t PL_J>
 __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
 {
     constexpr uint32_t PL_TILE = pl_tile(PL_J);
+    constexpr uint32_t F = LB_WIN | LB_OVF | LB_SUM;
     __shared__ uint32_t s_wcnt[PL_WAVES], s_wovf[PL_WAVES];
     __shared__ uint64_t s_wsum[PL_WAVES];
     __shared__ uint64_t s_P, s_carry;
-    __shared__ uint32_t s_epoch;
+    __shared__ uint32_t s_epoch, s_abort;
     __shared__ uint64_t s_vals[pl_tile(PL_J)];
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_epoch = ld32(&a.hdr->epoch);
-    __syncthreads();
-    const uint32_t epoch = s_epoch;
-    {
-        const uint64_t per = (a.cap + LB_CLEAR_SLICES - 1) / LB_CLEAR_SLICES;
-        uint64_t lo = (uint64_t)(epoch % LB_CLEAR_SLICES) * per;
-        const uint64_t hi = lo + per < a.cap ? lo + per : a.cap;
-        if (lo < a.total_tiles) lo = a.total_tiles;
-        for (uint64_t i = lo + (uint64_t)blockIdx.x * 256 + tid; i < hi; i += (uint64_t)gridDim.x * 256)
-            for (uint32_t k = 0; k < PL_DESC_WORDS; k++) st_rel(&a.desc[i * PL_DESC_WORDS + k], 0);
+    const uint32_t g = blockIdx.x; // global tile (one per block, in order)
+    const PipeJob& J = a.jobs[pl_find(a.jobs, a.njobs, g)];
+    const uint32_t t = g - J.tile0;
+    if (tid == 0) {
+        s_epoch = ld32(&a.hdr->epoch);
+        s_abort = ld32(&a.hdr->abort);
     }
-    uint32_t g = blockIdx.x;
-    if (g >= a.total_tiles) return;
+    __syncthreads();
+    if (s_abort) {
+        if (t == J.T - 1 && tid == 0 && J.stats) J.stats[0] = STATS_INVALID;
+        return;
+    }
+    const uint64_t n = J.a.n;
+    const uint32_t bo = J.s.bits;
+    const uint64_t om = J.s.mask, vm = lowmask(J.vbits);
+    const uint64_t wbase = (uint64_t)t * PL_TILE + wave * (PL_J * 64);
+    const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < PL_J * 64 ? n - wbase : PL_J * 64) : 0;
+    const TileReader<IK> ra(J.a, wbase < n ? wbase : 0), rb(J.b, wbase < n ? wbase : 0);
+    auto idx = [&](uint32_t j) { return j * 64 + lane < wrem ? j * 64 + lane : (wrem ? wrem - 1 : 0); };
 
+    // ---- 1. select, add, count, sum
     Fetch<IK, false> fa[PL_J], fb[PL_J];
-    auto fetch_tile = [&](uint32_t gt, Fetch<IK, false>* xa, Fetch<IK, false>* xb) {
-        const PipeJob& J = a.jobs[pl_find(a.jobs, a.njobs, gt)];
-        const uint64_t n = J.a.n;
-        const uint64_t wb = (uint64_t)(gt - J.tile0) * PL_TILE + wave * (PL_J * 64);
-        if (wb < n) {
-            const uint32_t rem = (uint32_t)(n - wb < PL_J * 64 ? n - wb : PL_J * 64);
-            TileReader<IK> ra(J.a, wb), rb(J.b, wb);
-#pragma unroll
-            for (uint32_t j = 0; j < PL_J; j++) {
-                const uint32_t i = j * 64 + lane < rem ? j * 64 + lane : rem - 1;
-                xa[j].load(ra, i);
-                xb[j].load(rb, i);
-            }
-        }
-    };
-    fetch_tile(g, fa, fb);
-
-    for (; g < a.total_tiles; g += gridDim.x) {
-        Fetch<IK, false> na[PL_J], nb[PL_J];
-        if (g + gridDim.x < a.total_tiles) fetch_tile(g + gridDim.x, na, nb);
-
-        const PipeJob& J = a.jobs[pl_find(a.jobs, a.njobs, g)];
-        const uint32_t t = g - J.tile0;
-        const uint64_t n = J.a.n;
-        const uint32_t bo = J.s.bits;
-        const uint64_t om = J.s.mask, vm = lowmask(J.vbits);
-        const uint64_t wbase = (uint64_t)t * PL_TILE + wave * (PL_J * 64);
-        const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < PL_J * 64 ? n - wbase : PL_J * 64) : 0;
-
-        // ---- 1. select, add, count, sum
-        uint64_t mine = 0, vsum = 0;
-        uint32_t cnt = 0, ovf = 0;
-        uint64_t val[PL_J];
+    if (wrem) {
 #pragma unroll
         for (uint32_t j = 0; j < PL_J; j++) {
-            const uint64_t x = fa[j].value(J.a.mask), y = fb[j].value(J.b.mask);
-            const bool fl = (j * 64 + lane < wrem) && J.pr.sel32((uint32_t)x);
-            val[j] = (x + y) & vm;
-            const uint64_t bl = ballot(fl);
-            if (lane == j) mine = bl;
-            cnt += (uint32_t)__popcll(bl);
-            vsum += fl ? val[j] : 0;
-            if (J.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && (val[j] & ~om) != 0));
+            fa[j].load(ra, idx(j));
+            fb[j].load(rb, idx(j));
         }
-        vsum = wave_sum_u64(vsum);
+    }
+    uint64_t mine = 0, vsum = 0;
+    uint32_t cnt = 0, ovf = 0;
+    uint64_t val[PL_J];
+#pragma unroll
+    for (uint32_t j = 0; j < PL_J; j++) {
+        const uint64_t x = wrem ? fa[j].value(ra, idx(j)) : 0, y = wrem ? fb[j].value(rb, idx(j)) : 0;
+        const bool fl = (j * 64 + lane < wrem) && J.pr.sel32((uint32_t)x);
+        val[j] = (x + y) & vm;
+        const uint64_t bl = ballot(fl);
+        if (lane == j) mine = bl;
+        cnt += (uint32_t)__popcll(bl);
+        vsum += fl ? val[j] : 0;
+        if (J.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && (val[j] & ~om) != 0));
+    }
+    vsum = wave_sum_u64(vsum);
+    if (lane == 0) {
+        s_wcnt[wave] = cnt;
+        s_wovf[wave] = ovf;
+        s_wsum[wave] = vsum;
+    }
+    __syncthreads();
+    uint32_t wpre = 0, ctile = 0, otile = 0;
+    uint64_t stile = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < PL_WAVES; w++) {
+        wpre += w < wave ? s_wcnt[w] : 0;
+        ctile += s_wcnt[w];
+        otile += s_wovf[w];
+        stile += s_wsum[w];
+    }
+    {
+        uint32_t run = wpre;
+#pragma unroll
+        for (uint32_t j = 0; j < PL_J; j++) {
+            const uint64_t bl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
+            if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = val[j] & om;
+            run += (uint32_t)__popcll(bl);
+        }
+    }
+    __syncthreads();
+
+    // ---- 2. look-back within the column (wave 0)
+    if (wave == 0) {
+        uint64_t own_w = 0;
+        if (bo < 64) {
+            const uint32_t k = (63 + bo - 1) / bo;
+            const uint32_t take = ctile < k ? ctile : k;
+            if (lane < take) {
+                const uint32_t jx = ctile - take + lane;
+                const int64_t shift = (int64_t)jx * bo - ((int64_t)ctile * bo - 63);
+                const uint64_t v = s_vals[jx];
+                own_w = shift >= 0 ? (v << shift) : (v >> (-shift));
+            }
+            own_w = wave_or_u64(own_w) & WIN_BITS;
+        }
+        Lookback<F> L{};
+        L.epoch = s_epoch;
+        L.bits = bo;
+        L.hdr = a.hdr;
+        L.tiles = a.lb_tiles + (uint64_t)J.tile0 * Lookback<F>::TS;
+        L.groups = a.lb_groups + (uint64_t)J.gbase * Lookback<F>::GS;
+        const Agg own{ctile, own_w, otile, stile};
+        const Agg ex = L.run(t, own, lane);
         if (lane == 0) {
-            s_wcnt[wave] = cnt;
-            s_wovf[wave] = ovf;
-            s_wsum[wave] = vsum;
-        }
-        __syncthreads();
-        uint32_t wpre = 0, ctile = 0, otile = 0;
-        uint64_t stile = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < PL_WAVES; w++) {
-            wpre += w < wave ? s_wcnt[w] : 0;
-            ctile += s_wcnt[w];
-            otile += s_wovf[w];
-            stile += s_wsum[w];
-        }
-        {
-            uint32_t run = wpre;
-#pragma unroll
-            for (uint32_t j = 0; j < PL_J; j++) {
-                const uint64_t bl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
-                                    (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
-                if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = val[j] & om;
-                run += (uint32_t)__popcll(bl);
+            s_P = ex.c;
+            s_carry = ex.w;
+            if (t == J.T - 1 && J.stats) {
+                J.stats[0] = L.aborted() ? STATS_INVALID : ex.c + ctile;
+                J.stats[1] = ovf_add(ex.o, otile);
+                J.stats[2] = ex.s + stile;
+                J.stats[3] = 0;
             }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    const uint64_t P = s_P, carry = s_carry;
+    const bool last = (t == J.T - 1);
+    const uint64_t ocap = J.s.n;
 
-        // ---- 2. look-back within the column (wave 0)
-        if (wave == 0) {
-            uint64_t own_w = 0;
-            if (bo < 64) {
-                const uint32_t k = (63 + bo - 1) / bo;
-                const uint32_t take = ctile < k ? ctile : k;
-                if (lane < take) {
-                    const uint32_t jx = ctile - take + lane;
-                    const int64_t shift = (int64_t)jx * bo - ((int64_t)ctile * bo - 63);
-                    const uint64_t v = s_vals[jx];
-                    own_w = shift >= 0 ? (v << shift) : (v >> (-shift));
-                }
-                own_w = wave_or_u64(own_w) & WIN_BITS;
+    // ---- 3. write the words this tile completes
+    if (J.s.kind == K_SLAB) {
+        const uint64_t eps = J.s.eps;
+        const uint64_t kf = P / eps;
+        uint64_t kl = (P + ctile) / eps;
+        if (last && ((P + ctile) % eps)) kl += 1;
+        const uint64_t wcap = (ocap + eps - 1) / eps;
+        if (kl > wcap) kl = wcap;
+        for (uint64_t k = kf + tid; k < kl; k += 256) {
+            uint64_t word = 0;
+            const uint64_t e0 = k * eps;
+            for (uint32_t s = 0; s < eps; s++) {
+                const uint64_t e = e0 + s;
+                if (e >= ocap) break;
+                uint64_t v;
+                if (e < P) v = (carry >> (63 - (P - e) * bo)) & om;
+                else if (e < P + ctile) v = s_vals[e - P];
+                else break;
+                word |= v << (s * bo);
             }
-            uint64_t* D = a.desc + (uint64_t)g * PL_DESC_WORDS;
-            uint64_t ec = 0, ew = 0, eo = 0, es = 0;
-            if (t > 0) {
-                if (lane == 0) {
-                    st_rel(&D[1], lb_word(1, epoch, own_w));
-                    st_rel(&D[2], lb_word(1, epoch, (own_w >> 46) | ((uint64_t)otile << 17)));
-                    st_rel(&D[3], lb_word(1, epoch, stile));
-                    st_rel(&D[4], lb_word(1, epoch, stile >> 46));
-                    st_rel(&D[0], lb_status(1, epoch, ctile));
-                }
-                int64_t pos = (int64_t)t - 1;
-                while (true) {
-                    const int64_t idx = pos - (int64_t)lane; // column-local tile
-                    uint64_t s = 0;
-                    uint32_t lvl = 2;
-                    const uint64_t* Q = a.desc + ((uint64_t)J.tile0 + (idx >= 0 ? idx : 0)) * PL_DESC_WORDS;
-                    if (idx >= 0) {
-                        while (!lb_status_ok(s = ld_acq(&Q[0]), epoch)) __builtin_amdgcn_s_sleep(1);
-                        lvl = (uint32_t)(s >> 62);
-                    }
-                    const uint64_t pm = ballot(lvl == 2);
-                    const uint32_t stop = pm ? (uint32_t)__builtin_ctzll(pm) : 64;
-                    const bool incl = lane <= stop && idx >= 0;
-                    uint64_t c = 0, w = 0, o = 0, sm = 0;
-                    if (incl) {
-                        c = s & CNT_MASK;
-                        const uint64_t* W = Q + (lvl == 2 ? 5 : 1);
-                        uint64_t x[4];
-                        for (uint32_t k = 0; k < 4; k++)
-                            while (!lb_word_ok(x[k] = ld_acq(&W[k]), lvl, epoch)) __builtin_amdgcn_s_sleep(1);
-                        w = (x[0] & P46) | ((x[1] & 0x1FFFF) << 46);
-                        o = (x[1] & P46) >> 17;
-                        sm = (x[2] & P46) | ((x[3] & P46) << 46);
-                    }
-                    for (int m = 1; m < 64; m <<= 1) {
-                        const uint64_t oc = shfl_xor_u64(c, m), ow = shfl_xor_u64(w, m);
-                        const uint64_t oo = shfl_xor_u64(o, m), os = shfl_xor_u64(sm, m);
-                        if (lane & m) w = win_combine(w, ow, oc, bo);
-                        else w = win_combine(ow, w, c, bo);
-                        c += oc;
-                        o = ovf_add(o, oo);
-                        sm += os;
-                    }
-                    ew = win_combine(w, ew, ec, bo);
-                    ec += c;
-                    eo = ovf_add(eo, o);
-                    es += sm;
-                    if (pm) break;
-                    pos -= 64;
-                }
-            }
-            const uint64_t ic = ec + ctile, iw = win_combine(ew, own_w, ctile, bo);
-            const uint64_t io = ovf_add(eo, otile), is = es + stile;
-            if (lane == 0) {
-                st_rel(&D[5], lb_word(2, epoch, iw));
-                st_rel(&D[6], lb_word(2, epoch, (iw >> 46) | (io << 17)));
-                st_rel(&D[7], lb_word(2, epoch, is));
-                st_rel(&D[8], lb_word(2, epoch, is >> 46));
-                st_rel(&D[0], lb_status(2, epoch, ic));
-                s_P = ec;
-                s_carry = ew;
-                if (t == J.T - 1 && J.stats) {
-                    J.stats[0] = ic;
-                    J.stats[1] = io;
-                    J.stats[2] = is;
-                    J.stats[3] = 0;
-                }
-            }
+            J.s.p[k] = word;
         }
-        __syncthreads();
-        const uint64_t P = s_P, carry = s_carry;
-        const bool last = (t == J.T - 1);
-        const uint64_t ocap = J.s.n;
-
-        // ---- 3. write the words this tile completes
-        if (J.s.kind == K_SLAB) {
-            const uint64_t eps = J.s.eps;
-            const uint64_t kf = P / eps;
-            uint64_t kl = (P + ctile) / eps;
-            if (last && ((P + ctile) % eps)) kl += 1;
-            const uint64_t wcap = (ocap + eps - 1) / eps;
-            if (kl > wcap) kl = wcap;
-            for (uint64_t k = kf + tid; k < kl; k += 256) {
-                uint64_t word = 0;
-                const uint64_t e0 = k * eps;
-                for (uint32_t s = 0; s < eps; s++) {
-                    const uint64_t e = e0 + s;
-                    if (e >= ocap) break;
-                    uint64_t v;
-                    if (e < P) v = (carry >> (63 - (P - e) * bo)) & om;
-                    else if (e < P + ctile) v = s_vals[e - P];
-                    else break;
-                    word |= v << (s * bo);
-                }
-                J.s.p[k] = word;
+    }
+    else {
+        const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
+        const uint64_t kf = B0 >> 6;
+        uint64_t kl = B1 >> 6;
+        if (last && (B1 & 63)) kl += 1;
+        const uint64_t wcap = (Bcap + 63) >> 6;
+        if (kl > wcap) kl = wcap;
+        for (uint64_t k = kf + tid; k < kl; k += 256) {
+            const uint64_t b0 = k * 64;
+            uint64_t word = 0;
+            if (b0 < B0) word = carry >> (b0 + 63 - B0);
+            uint64_t e0 = b0 / bo;
+            if (e0 < P) e0 = P;
+            uint64_t e1 = (b0 + 63) / bo + 1;
+            if (e1 > P + ctile) e1 = P + ctile;
+            for (uint64_t e = e0; e < e1; e++) {
+                const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
+                const uint64_t v = s_vals[e - P];
+                word |= pos >= 0 ? (v << pos) : (v >> (-pos));
             }
-        }
-        else {
-            const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
-            const uint64_t kf = B0 >> 6;
-            uint64_t kl = B1 >> 6;
-            if (last && (B1 & 63)) kl += 1;
-            const uint64_t wcap = (Bcap + 63) >> 6;
-            if (kl > wcap) kl = wcap;
-            for (uint64_t k = kf + tid; k < kl; k += 256) {
-                const uint64_t b0 = k * 64;
-                uint64_t word = 0;
-                if (b0 < B0) word = carry >> (b0 + 63 - B0);
-                uint64_t e0 = b0 / bo;
-                if (e0 < P) e0 = P;
-                uint64_t e1 = (b0 + 63) / bo + 1;
-                if (e1 > P + ctile) e1 = P + ctile;
-                for (uint64_t e = e0; e < e1; e++) {
-                    const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
-                    const uint64_t v = s_vals[e - P];
-                    word |= pos >= 0 ? (v << pos) : (v >> (-pos));
-                }
-                if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-                J.s.p[k] = word;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t j = 0; j < PL_J; j++) {
-            fa[j] = na[j];
-            fb[j] = nb[j];
+            if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
+            J.s.p[k] = word;
         }
     }
 }
@@ -316,14 +245,26 @@ __global__ void k_epoch_bump(WsHeader* hdr)
 
 using namespace am;
 
-// Workspace: header + small region, then the job table, then 9 descriptor
-// words per 4096-element tile.
+// Workspace: header + small region, job table, then the look-back area:
+// tile descriptors for every tile of the batch and (tiles/64 + 1) group
+// descriptors per column.
+using PlLookback = Lookback<LB_WIN | LB_OVF | LB_SUM>;
+
+static uint64_t pl_lb_bytes(uint64_t tiles, uint64_t groups)
+{
+    return tiles * PlLookback::TS * 8 + groups * PlLookback::GS * 8;
+}
+
 extern "C" uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t count)
 {
     const uint32_t PL_TILE = pl_tile(8); // the smallest variant bounds the tile count
-    uint64_t tiles = 1;
-    for (uint32_t i = 0; i < count; i++) tiles += n[i] ? (n[i] + PL_TILE - 1) / PL_TILE : 1;
-    return 256 + WS_SMALL_BYTES + ((uint64_t)count * sizeof(PipeJob) + 255) / 256 * 256 + tiles * PL_DESC_WORDS * 8;
+    uint64_t tiles = 1, groups = 1;
+    for (uint32_t i = 0; i < count; i++) {
+        const uint64_t t = n[i] ? (n[i] + PL_TILE - 1) / PL_TILE : 1;
+        tiles += t;
+        groups += t / 64 + 1;
+    }
+    return 256 + WS_SMALL_BYTES + ((uint64_t)count * sizeof(PipeJob) + 255) / 256 * 256 + pl_lb_bytes(tiles, groups);
 }
 
 // Host-side validation + job table upload (synchronous); after this,
@@ -338,7 +279,7 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
     for (uint32_t i = 0; i < count; i++) ns[i] = cols[i].a.n;
     if (ws_bytes < am_pipeline_workspace_bytes(ns.data(), count)) return AM_E_WORKSPACE;
     std::vector<PipeJob> jobs(count);
-    uint64_t tiles = 0;
+    uint64_t tiles = 0, groups = 0;
     int kind0 = -1;
     for (uint32_t i = 0; i < count; i++) {
         const am_pipeline_col& c = cols[i];
@@ -359,12 +300,17 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
         J.chk_ovf = c.s.bits < J.vbits ? 1 : 0;
         J.T = J.a.n ? (uint32_t)((J.a.n + PL_TILE - 1) / PL_TILE) : 1;
         J.tile0 = (uint32_t)tiles;
+        J.gbase = (uint32_t)groups;
         tiles += J.T;
+        groups += J.T / 64 + 1;
         if (tiles >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
     }
     uint8_t* jt = (uint8_t*)ws + 256 + WS_SMALL_BYTES;
     if (hipMemcpy(jt, jobs.data(), count * sizeof(PipeJob), hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
     if (total_tiles) *total_tiles = (uint32_t)tiles;
+    // the group area starts right after the tile area of this batch
+    uint64_t meta[2] = {tiles, groups};
+    if (hipMemcpy((uint8_t*)ws + 32, meta, sizeof(meta), hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
     return AM_OK;
 }
 
@@ -378,18 +324,15 @@ extern "C" int am_pipeline_run(uint32_t count, uint32_t total_tiles, uint32_t in
     a.total_tiles = total_tiles;
     a.hdr = (WsHeader*)ws;
     const uint64_t off = 256 + WS_SMALL_BYTES + ((uint64_t)count * sizeof(PipeJob) + 255) / 256 * 256;
-    if (ws_bytes < off + (uint64_t)(total_tiles + 1) * PL_DESC_WORDS * 8) return AM_E_WORKSPACE;
-    a.desc = (uint64_t*)((uint8_t*)ws + off);
-    a.cap = (ws_bytes - off) / (PL_DESC_WORDS * 8);
+    if (ws_bytes < off + pl_lb_bytes(total_tiles + 1, 1)) return AM_E_WORKSPACE;
+    // Every launch of a prepared batch rewrites all of its tile descriptors,
+    // and group aggregates are validated by their status word's epoch, so
+    // nothing here needs slice clearing.
+    a.lb_tiles = (uint64_t*)((uint8_t*)ws + off);
+    a.lb_groups = a.lb_tiles + (uint64_t)total_tiles * PlLookback::TS;
     hipStream_t s = (hipStream_t)stream;
-#define PL_GO(IK, JJ)                                                                    \
-    {                                                                                    \
-        static uint32_t capb = 0;                                                        \
-        if (!capb) capb = resident_blocks(k_pipeline<IK, JJ>, 256, 0);                   \
-        const uint32_t gsz = total_tiles < capb ? total_tiles : capb;                    \
-        hipLaunchKernelGGL((k_pipeline<IK, JJ>), dim3(gsz), dim3(256), 0, s, a);         \
-    }
-#define PL_V(IK) { if (variant == 8) PL_GO(IK, 8) else PL_GO(IK, 16) }
+#define PL_GO(IK, JJ) hipLaunchKernelGGL((k_pipeline<IK, JJ>), dim3(total_tiles), dim3(256), 0, s, a);
+#define PL_V(IK) { if (variant == 8) { PL_GO(IK, 8) } else { PL_GO(IK, 16) } }
     switch (in_format) {
         case AM_FMT_SLAB: PL_V(K_SLAB); break;
         case AM_FMT_BITSTREAM: PL_V(K_BITSTREAM); break;

@@ -200,6 +200,141 @@ def sweep(am, torch, timer, reps):
     return ops
 
 
+def groupby_leg(am, torch, timer, reps):
+    """256M rows: keys 10-bit Zipf(1.1) over 1024 ranks, values 20-bit uniform;
+    exact SUM + COUNT, minimal-width key-sorted output; plus the chained
+    filter (v > 2^19) variant."""
+    n = 1 << 28
+    p = 1.0 / np.arange(1, 1025) ** 1.1
+    c = np.cumsum(p / p.sum())
+    thr = np.minimum(np.floor(c * 4294967296.0), 4294967295).astype(np.uint64)
+    thr[-1] = 4294967295
+    keys = am.gen_from_cdf(42, thr.astype(np.uint32), n, 10, am.SLAB)
+    vals = am.gen_uniform(43, n, 20, am.SLAB)
+    ws = am.Workspace(0)
+    out = []
+    for label, pred in (("groupby sum+count", None), ("filter(v>2^19)->groupby", (am.GT, 1 << 19))):
+        ms = timer.median_ms(lambda: am.groupby(keys, vals, value_pred=pred, ws=ws), reps)
+        _, _, _, res = am.groupby(keys, vals, value_pred=pred, ws=ws)
+        torch.cuda.synchronize()
+        r = am.u64(res)
+        g = int(r[0])
+        byt = keys.words * 8 + vals.words * 8 + g * 8 * 3
+        out.append({"op": label, "rows": n, "groups": g, "sum_bits": int(r[1]), "count_bits": int(r[2]),
+                    "ms": round(ms, 4), "GBs": round(gbytes(byt, ms), 1),
+                    "pct_peak": round(100 * gbytes(byt, ms) / PEAK_GBS, 1)})
+    return out
+
+
+def matmul_leg(am, torch, timer, reps):
+    """A, B 8192x8192 7-bit (slab packed), exact i32 via int8 MFMA, C re-packed
+    at 27 bits; plus 256 uneven matrices (M, N, K ~ U[64, 2048]) in one batch."""
+    M = N = K = 8192
+    bits = 7
+    words = am.matrix_words(M, K, bits)
+    A = am.gen_uniform(5, words * 9, 7, am.SLAB)  # any 7-bit slab words of the right count
+    B = am.gen_uniform(6, words * 9, 7, am.SLAB)
+    prob = [(A.buf, B.buf, M, N, K)]
+    outs = am.matmul_packed_batched(bits, prob)
+    torch.cuda.synchronize()
+    ms = timer.median_ms(lambda: am.matmul_packed_batched(bits, prob), max(2, reps // 3), warm=1)
+    tops = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+    res = [{"op": "matmul u7 8192^3 (unpack+gemm+pack)", "ms": round(ms, 3), "TOPS": round(tops, 1),
+            "pct_int8_peak": round(100 * tops / 5000.0, 1), "out_bits": outs[0][2]}]
+    rng = np.random.default_rng(7)
+    probs = []
+    flops = 0
+    for i in range(256):
+        m, nn, k = (int(x) for x in rng.integers(64, 2049, 3))
+        wa, wb = am.matrix_words(m, k, bits), am.matrix_words(k, nn, bits)
+        a = am.gen_uniform(100 + i, wa * 9, 7, am.SLAB)
+        b = am.gen_uniform(1000 + i, wb * 9, 7, am.SLAB)
+        probs.append((a.buf, b.buf, m, nn, k))
+        flops += 2.0 * m * nn * k
+    am.matmul_packed_batched(bits, probs)
+    torch.cuda.synchronize()
+    ms = timer.median_ms(lambda: am.matmul_packed_batched(bits, probs), max(2, reps // 3), warm=1)
+    tops = flops / (ms * 1e-3) / 1e12
+    res.append({"op": "matmul u7 batch of 256 uneven", "ms": round(ms, 3), "TOPS": round(tops, 1),
+                "pct_int8_peak": round(100 * tops / 5000.0, 1)})
+    return res
+
+
+def batched_leg(am, torch, reps):
+    """512 columns, lengths log-uniform in [4K, 64M], widths U[3,31], values
+    uniform / Zipf(1.2) / sorted+noise; per column filter -> compressstore ->
+    add second column -> sum. Plain: one operator launch at a time per column
+    (count read back between stages). Batched: planner (HIP features + MLP, or
+    the analytic plan without trained weights) then one persistent launch per
+    kernel variant, replayed from a captured graph."""
+    from amethyst_amd import planner
+    cols = []
+    for i in range(512):
+        spec = planner.column_spec(50_000 + i)
+        cols.append(planner.make_inputs(50_000 + i, spec))
+    torch.cuda.synchronize()
+    rows = sum(c[0].n for c in cols)
+    # plain plan
+    planner.run_plain(cols)
+    t0 = time.perf_counter()
+    plain_totals = planner.run_plain(cols)
+    t_plain = time.perf_counter() - t0
+    # planning
+    t0 = time.perf_counter()
+    feat = planner.features(cols)
+    try:
+        pl = planner.Planner()
+        plans, _ = pl.choose(feat)
+        kind = "learned"
+    except (OSError, FileNotFoundError):
+        plans = np.full(len(cols), planner.ANALYTIC, dtype=np.int32)
+        kind = "analytic (no trained weights)"
+    t_plan = time.perf_counter() - t0
+    fr = feat.cpu().numpy()
+    outs, stats, t_batch, redo = planner.run_batch(cols, list(plans), fr, reps=max(3, reps // 2))
+    ok = [int(s[2]) for s in stats] == plain_totals
+    # graph replay of the batched launches (same plans, no fallback needed now)
+    groups = {}
+    for i, p in enumerate(plans):
+        groups.setdefault(planner.PLANS[p][2], []).append(i)
+    batches = []
+    for var, idx in groups.items():
+        spec = []
+        for i in idx:
+            a, b, pred = cols[i]
+            fmt, wr, _ = planner.PLANS[plans[i]]
+            w = planner.plan_width(wr, a.bits, planner.est_bits(fr[i]))
+            if int(stats[i][1]):
+                w = a.bits + 1
+            spec.append((a, b, pred, fmt, w))
+        batches.append(am.PipelineBatch(spec, variant=var))
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for bt in batches:
+            bt.run()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        for bt in batches:
+            bt.run()
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nrep = max(3, reps // 2)
+    for _ in range(nrep):
+        g.replay()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / nrep
+    res_ok = all(int(bt.results()[1][j][2]) == plain_totals[i] for bt, (var, idx) in zip(batches, groups.items())
+                 for j, i in enumerate(idx))
+    return {"columns": len(cols), "rows": rows, "plain_s": round(t_plain, 5), "batched_s": round(t_batch, 5),
+            "graph_s": round(t_graph, 5), "plan_s": round(t_plan, 5), "planner": kind,
+            "speedup_batched": round(t_plain / t_batch, 2), "speedup_graph": round(t_plain / t_graph, 2),
+            "fallback_reruns": redo, "totals_match_plain": bool(ok and res_ok)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -230,8 +365,10 @@ def main():
     def step():
         am.compressstore(col, 32, am.U32, pred=(am.LT, c), ws=ws, out=out, stats=stats, mask_out=mask_out)
 
+    print(f"headline: compressstore n={n} w={w}", file=sys.stderr, flush=True)
     step()
     torch.cuda.synchronize()
+    ws.check()
     k = int(am.u64(stats)[0])
     byt = col.words * 8 + 4 * k + ((n + 63) // 64) * 8
     for _ in range(args.warmup):
@@ -247,6 +384,7 @@ def main():
         b_.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    ws.check()
     kms = [a_.elapsed_time(b_) for a_, b_ in evs]
     ms_step = wall * 1e3 / args.steps
     avg_kernel = sum(kms) / len(kms)
@@ -255,6 +393,14 @@ def main():
     traffic = read_traffic("k_compress_w11_u32_traffic.txt")
 
     ops = [] if args.quick else sweep(am, torch, timer, args.reps)
+    extra = {}
+    if not args.quick:
+        print("group-by", file=sys.stderr, flush=True)
+        ops += groupby_leg(am, torch, timer, args.reps)
+        print("matmul", file=sys.stderr, flush=True)
+        ops += matmul_leg(am, torch, timer, args.reps)
+        print("batched", file=sys.stderr, flush=True)
+        extra["batched"] = batched_leg(am, torch, args.reps)
     print("cpu baseline", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(am, torch, col, c)
 
@@ -280,6 +426,7 @@ def main():
         "cpu_baseline": cpu,
         "ops": ops,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

@@ -54,7 +54,9 @@ typedef enum {
     AM_E_UNSUPPORTED_CONFIG = 2, /* replaces the reference's -1 "skip" return */
     AM_E_HIP = 3,                /* a HIP runtime call failed */
     AM_E_WORKSPACE = 4,          /* workspace missing or too small */
-    AM_E_CAPACITY = 5            /* output buffer too small for the request */
+    AM_E_CAPACITY = 5,           /* output buffer too small for the request */
+    AM_E_ABORTED = 6             /* a look-back wait exceeded its time limit; the
+                                    workspace is poisoned until re-initialised */
 } am_status;
 
 typedef enum {
@@ -142,6 +144,14 @@ uint64_t am_gen_bitmask(uint32_t bits);
  * share one workspace must be ordered on one stream. */
 uint64_t am_workspace_bytes(uint64_t max_elements);
 int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream);
+
+/* Synchronous health check of a workspace: AM_OK, or AM_E_ABORTED when a
+ * single-pass launch gave up waiting on a predecessor tile (every wait is
+ * bounded at 2 s; such a launch writes statistics[0] = UINT64_MAX and later
+ * launches on the poisoned workspace do nothing until am_workspace_init).
+ * No reference counterpart: the reference's multi-launch operators do not
+ * wait across blocks. */
+int am_workspace_status(const void* ws);
 
 /* ---------------------------------------------------------------- data generation */
 

@@ -131,12 +131,15 @@ def test_compress_mask_formats(am, oracle_mod):
                 mbits = rng.random(n) < sel
                 mask = np.packbits(mbits, bitorder="little")
                 mask = np.concatenate([mask, np.zeros((-len(mask)) % 8, np.uint8)]).view(np.uint64)
-                for fi in (0, 1, 2):
+                for fi in (0, 1, 2, 3, 4):
+                    if not fmt_ok(fi, bits):
+                        continue
                     for fo, ob in ((1, bits), (2, bits), (0, bits), (4, min(bits, 32)), (1, max(1, bits - 1))):
                         if not fmt_ok(fo, ob):
                             continue
                         say("compress-mask", bits, n, sel, fi, fo, ob)
                         _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, mask_words=mask, ws=ws)
+    ws.check()
 
 
 def test_compress_predicate_fused(am, oracle_mod):
@@ -151,6 +154,39 @@ def test_compress_predicate_fused(am, oracle_mod):
             for fo, ob in ((4, bits), (1, bits), (2, bits), (1, pw), (2, pw), (3, pw)):
                 say("compress-pred", bits, frac, fo, ob)
                 _check_compress(am, oracle_mod, vals, bits, 1, ob, fo, pred=(0, c), ws=ws)
+    ws.check()
+
+
+def test_compress_predicate_all_inputs(am, oracle_mod):
+    """Predicate compaction from every input format and width class (word
+    kernel for slab/containers, element kernel for bitstream), every
+    predicate operator."""
+    rng = np.random.default_rng(14)
+    ws = am.Workspace(1 << 20)
+    for bits in (1, 3, 5, 8, 9, 16, 21, 31, 32, 33, 40, 64):
+        for n in (1, 1023, 1024, 1025, 65, 70001):
+            vals = rand_vals(rng, n, bits)
+            c = int(vals[n // 2])
+            for fi in FORMATS:
+                if not fmt_ok(fi, bits):
+                    continue
+                for op in range(6):
+                    fo, ob = ((4, min(bits, 32)), (1, bits), (2, bits), (0, bits), (3, bits), (1, max(1, bits // 2)))[op]
+                    say("compress-pred-in", bits, n, fi, op, fo, ob)
+                    _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, pred=(op, c), ws=ws)
+    ws.check()
+
+
+def test_compress_many_tiles(am, oracle_mod):
+    """More than 64*64 tiles: the group look-back takes several steps."""
+    rng = np.random.default_rng(15)
+    n = (1 << 25) + 12345
+    ws = am.Workspace(n)
+    for bits, fi, fo, ob in ((11, 1, 4, 11), (11, 1, 1, 10), (30, 4, 2, 30), (7, 2, 0, 7)):
+        vals = rand_vals(rng, n, bits)
+        say("compress-many-tiles", bits, fi, fo, ob)
+        _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, pred=(0, 1 << (bits - 1)), ws=ws)
+    ws.check()
 
 
 def test_compress_capacity_truncation(am, oracle_mod):
@@ -171,6 +207,7 @@ def test_compress_workspace_reuse(am, oracle_mod):
         vals = rand_vals(rng, n, 11)
         say("compress-reuse", i, n)
         _check_compress(am, oracle_mod, vals, 11, 1, 11, 2, pred=(0, 700), ws=ws)
+    ws.check()
 
 
 def test_binary_ops(am, oracle_mod):
