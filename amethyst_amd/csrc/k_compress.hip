@@ -13,8 +13,8 @@
 //      gives every lane whole 64-bit input words, decodes all fields of a
 //      word in registers and forms the word's selection bits (predicate
 //      evaluated in place, or the bits of the input mask); per 64-word chunk
-//      a wave scan gives each lane its rank, and the chunk's eps output mask
-//      words are gathered from the lanes' bit groups by shuffles. The element
+//      a DPP wave scan gives each lane its rank, and the lanes' bit groups
+//      are OR-ed into the tile's output mask words in LDS. The element
 //      kernel (bitstream inputs) decodes element base + 64j + lane per lane
 //      and uses wave64 ballots, which are the mask words themselves;
 //   2. staging its selected values, compacted, in LDS at the narrowest
@@ -36,7 +36,9 @@
 namespace am {
 
 constexpr uint32_t CS_THREADS = 256;          // 4 waves
-constexpr uint32_t CW_RMAX = 4;               // word kernel: 64-word chunks per wave (max)
+constexpr uint32_t CW_RMAX = 4;               // word kernel (packed outputs): 64-word chunks per wave (max)
+constexpr uint32_t CD_RMAX = 8;               // direct kernel (container outputs): 64-word chunks per wave (max)
+constexpr uint32_t CD_WORDS = 4 * 64 * CD_RMAX; // direct kernel: input tile words staged in LDS
 constexpr uint32_t CW_LDS = 16384;            // word kernel: bytes of staged values
 constexpr uint32_t CE_J = 16;                 // element kernel: elements per lane
 constexpr uint32_t CE_TILE = 4 * CE_J * 64;   // element kernel: 4096 elements per tile
@@ -62,6 +64,7 @@ struct CompressArgs {
     uint64_t* lb_tiles;
     uint64_t* lb_groups;
     uint64_t lb_cap;
+    uint32_t lds16; // input pointer 16-byte aligned: 16-byte global->LDS copies
 };
 
 __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
@@ -74,20 +77,22 @@ __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
     }
 }
 
-__device__ inline uint64_t shfl64(uint64_t v, uint32_t src)
+// Inclusive wave64 prefix sum with DPP: row_shr 1/2/4/8 within rows of 16
+// lanes (out-of-row sources read 0), then row_bcast 15 / 31 carry the row
+// totals into the following rows.
+template <int CTRL, int ROWM, bool BC>
+__device__ inline uint32_t dpp_u32(uint32_t v)
 {
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src);
-    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src);
-    return ((uint64_t)hi << 32) | lo;
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xf, BC);
 }
-
-__device__ inline uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
+__device__ inline uint32_t wave_incl_scan(uint32_t v)
 {
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t o = (uint32_t)__shfl_up((int)v, d);
-        if (lane >= d) v += o;
-    }
+    v += dpp_u32<0x111, 0xf, true>(v);  // row_shr:1
+    v += dpp_u32<0x112, 0xf, true>(v);  // row_shr:2
+    v += dpp_u32<0x114, 0xf, true>(v);  // row_shr:4
+    v += dpp_u32<0x118, 0xf, true>(v);  // row_shr:8
+    v += dpp_u32<0x142, 0xa, false>(v); // row_bcast:15 -> rows 1, 3
+    v += dpp_u32<0x143, 0xc, false>(v); // row_bcast:31 -> rows 2, 3
     return v;
 }
 
@@ -102,25 +107,6 @@ __device__ inline uint64_t readlane64(uint64_t v, uint32_t j)
 {
     return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), j) << 32) |
            (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, j);
-}
-
-// Lane k < eps returns output mask word k of a 64-word chunk whose lane L
-// holds the selection bits of elements L*eps .. L*eps+eps-1 in g.
-__device__ inline uint64_t mask_gather(uint64_t g, uint32_t eps, uint32_t lane)
-{
-    if (eps == 1) return ballot(g & 1);
-    const uint32_t k = lane < eps ? lane : 0;
-    const uint32_t L0 = (64 * k) / eps;
-    const uint32_t nl = 63 / eps + 2;
-    uint64_t mw = 0;
-    for (uint32_t i = 0; i < nl; i++) {
-        const uint32_t L = L0 + i;
-        const uint64_t gl = eps <= 32 ? (uint64_t)(uint32_t)__shfl((int)(uint32_t)g, (int)(L & 63))
-                                      : shfl64(g, L & 63);
-        const int32_t off = (int32_t)(L * eps) - (int32_t)(64 * k);
-        if (L < 64 && off < 64) mw |= off >= 0 ? gl << off : gl >> (-off);
-    }
-    return mw;
 }
 
 // Window of a tile: the last 63 bits of its packed output stream.
@@ -252,25 +238,61 @@ struct CsFlags {
 
 // ------------------------------------------------------------ word kernel
 //
+// Packed (slab / bitstream) outputs from slab / container inputs.
 // Tile = 4 waves x R chunks x 64 words (R = 4, 2, 1 for eps <= 8, <= 16,
-// more), i.e. 256*R*eps elements, a multiple of 64, so every tile starts on
-// an output mask word.
+// <= 32), i.e. 256*R*eps elements, a multiple of 64, so every tile starts on
+// an output mask word. Lane L of chunk c holds input word c*64+L, i.e. the
+// elements (c*64+L)*eps .. +eps-1 of the tile; its selection bits g go to
+// bit (c*64+L)*eps of the tile's mask (LDS, atomic OR), its rank among the
+// tile's selected values is the chunk's prefix plus a wave scan of the
+// per-lane popcounts (DPP, no LDS round trips).
+constexpr uint32_t CW_MASKW = 256; // output mask words per tile (max: R=1, eps=64)
+
+// field s of an input word (lo, hi halves) at bit offset sh (< 64), 32-bit result
+__device__ inline uint32_t field32(uint32_t lo, uint32_t hi, uint32_t sh)
+{
+    return sh < 32 ? __builtin_amdgcn_alignbit(hi, lo, sh) : (hi >> (sh - 32));
+}
+
 template <int OK, bool PRED, bool OVF, int VB, bool WIDE>
 __global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
 {
+    static_assert(OK != K_STRIDED, "container outputs use k_compress_direct");
     constexpr uint32_t F = CsFlags<OK, OVF>::F;
+    constexpr uint32_t RM = CW_RMAX; // chunks per wave (max)
     using V = typename VT<VB>::T;
     __shared__ V s_vals[CW_LDS / VB];
-    __shared__ uint32_t s_ccnt[4 * CW_RMAX], s_covf[4 * CW_RMAX];
+    __shared__ uint64_t s_mask[CW_MASKW]; // output mask (mask_out) or input mask (mask_in) of the tile
+    __shared__ uint32_t s_ccnt[4 * RM], s_covf[4 * RM];
     __shared__ uint64_t s_P, s_carry;
     __shared__ uint32_t s_epoch, s_abort;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t t = blockIdx.x;
+    const uint32_t fw = a.fw, eps = a.eps, R = a.R;
+    const uint64_t n = a.in.n;
+    const uint64_t TW = (uint64_t)CS_THREADS * R; // words per tile
+    const uint64_t nmw = (n + 63) >> 6;
+    const uint32_t NMW = 4 * R * eps;              // mask words per tile (<= CW_MASKW)
+    const uint64_t mw0 = (t * TW * eps) >> 6;      // first mask word of the tile
+
+    // ---- loads first: input mask words, input words, then the epoch
+    uint64_t mval = 0;
+    if (!PRED && tid < NMW && mw0 + tid < nmw) mval = a.mask_in[mw0 + tid];
+    uint64_t x[RM];
+#pragma unroll
+    for (uint32_t r = 0; r < RM; r++) {
+        x[r] = 0;
+        if (r < R) {
+            const uint64_t wi = t * TW + (uint64_t)(wave * R + r) * 64 + lane;
+            if (wi < a.in_words) x[r] = a.in.p[wi];
+        }
+    }
     if (tid == 0) {
         s_epoch = ld32(&a.hdr->epoch);
         s_abort = ld32(&a.hdr->abort);
     }
+    if (tid < NMW) s_mask[tid] = PRED ? 0 : mval;
     __syncthreads();
     if (s_abort) {
         if (t == a.T - 1 && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
@@ -279,99 +301,319 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
     const Lookback<F> L = make_lookback<F>(a, s_epoch);
     L.clear_slice(a.T, t * CS_THREADS + tid, (uint64_t)a.T * CS_THREADS);
 
-    const uint32_t fw = a.fw, eps = a.eps, R = a.R;
-    const uint64_t n = a.in.n;
-    const uint64_t vm = a.in.mask & a.out.mask; // staged value
     const uint64_t om = a.out.mask;
-    const uint64_t TW = (uint64_t)CS_THREADS * R; // words per tile
-    const uint64_t nmw = (n + 63) >> 6;
-
-    uint64_t x[CW_RMAX], g[CW_RMAX], mk[CW_RMAX];
-    uint32_t lp[CW_RMAX];
+    const uint32_t m32 = (uint32_t)a.in.mask, vm32 = (uint32_t)(a.in.mask & om);
+    const uint64_t vm = a.in.mask & om;
+    uint32_t g[RM];
+    uint32_t lp[RM];
 #pragma unroll
-    for (uint32_t r = 0; r < CW_RMAX; r++) {
-        x[r] = 0;
-        mk[r] = 0;
-        if (r < R) {
-            const uint64_t wi = t * TW + (uint64_t)(wave * R + r) * 64 + lane;
-            if (wi < a.in_words) x[r] = a.in.p[wi];
-            if (!PRED && lane < eps) {
-                const uint64_t mw = ((t * TW + (uint64_t)(wave * R + r) * 64) * eps >> 6) + lane;
-                if (mw < nmw) mk[r] = a.mask_in[mw];
-            }
-        }
-    }
-#pragma unroll
-    for (uint32_t r = 0; r < CW_RMAX; r++) {
+    for (uint32_t r = 0; r < RM; r++) {
         g[r] = 0;
         lp[r] = 0;
         if (r >= R) continue;
-        const uint64_t w0 = t * TW + (uint64_t)(wave * R + r) * 64; // first word of the chunk
-        const uint64_t e0 = (w0 + lane) * eps;
+        const uint32_t ci = wave * R + r;
+        const uint64_t e0 = (t * TW + (uint64_t)ci * 64 + lane) * eps;
         const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
-        uint64_t gg = 0, go = 0;
+        const uint32_t p = (ci * 64 + lane) * eps; // bit of the tile mask
+        const uint32_t lo = (uint32_t)x[r], hi = (uint32_t)(x[r] >> 32);
+        uint32_t gg = 0;
+        uint32_t go = 0;
         if (PRED) {
-            for (uint32_t s = 0; s < eps; s++) {
-                const uint64_t v = (x[r] >> (s * fw)) & a.in.mask;
-                const bool sel = WIDE ? a.pr(v) : a.pr.sel32((uint32_t)v);
-                gg |= (uint64_t)sel << s;
-                if (OVF) go |= (uint64_t)(sel && (v & ~om) != 0) << s;
+            if (WIDE) {
+                const uint64_t v = x[r] & a.in.mask; // eps == 1
+                gg = a.pr(v);
+                if (OVF) go = gg && (v & ~om) != 0;
+            }
+            else {
+                uint32_t g32 = 0, o32 = 0;
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw) {
+                    const uint32_t v = field32(lo, hi, sh) & m32;
+                    const bool sel = a.pr.sel32(v);
+                    g32 |= (uint32_t)sel << s;
+                    if (OVF) o32 += sel && (v & ~(uint32_t)om) != 0;
+                }
+                gg = g32;
+                go = o32;
             }
         }
         else {
-            const uint32_t pos = lane * eps, k1 = pos >> 6, off = pos & 63;
-            const uint64_t lo = shfl64(mk[r], k1);
-            const uint64_t hi = shfl64(mk[r], (k1 + 1) & 63);
-            gg = (lo >> off) | ((off && off + eps > 64) ? hi << (64 - off) : 0);
-            if (OVF)
-                for (uint32_t s = 0; s < eps; s++) {
-                    const uint64_t v = (x[r] >> (s * fw)) & a.in.mask;
-                    go |= (uint64_t)((v & ~om) != 0) << s;
-                }
+            const uint32_t w = p >> 6, off = p & 63;
+            uint64_t m = s_mask[w] >> off;
+            if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
+            gg = (uint32_t)m;
+            if (OVF) {
+                if (WIDE) go = (gg & 1) && ((x[r] & a.in.mask) & ~om) != 0;
+                else
+                    for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                        go += ((gg >> s) & 1) && ((field32(lo, hi, sh) & m32) & ~(uint32_t)om) != 0;
+            }
         }
-        gg &= lowmask(nv);
+        gg &= (uint32_t)lowmask(nv);
         g[r] = gg;
-        const uint32_t c = (uint32_t)__popcll(gg);
-        const uint32_t inc = wave_incl_scan(c, lane);
+        const uint32_t c = (uint32_t)__popc(gg);
+        const uint32_t inc = wave_incl_scan(c);
         lp[r] = inc - c;
-        if (lane == 63) s_ccnt[wave * R + r] = inc;
+        if (lane == 63) s_ccnt[ci] = inc;
         if (OVF) {
-            const uint32_t oc = wave_sum32((uint32_t)__popcll(go & gg));
-            if (lane == 0) s_covf[wave * R + r] = oc;
+            const uint32_t oc = wave_sum32(go);
+            if (lane == 0) s_covf[ci] = oc;
         }
-        if (a.mask_out) {
-            const uint64_t mw = mask_gather(gg, eps, lane);
-            const uint64_t mi = ((w0 * eps) >> 6) + lane;
-            if (lane < eps && mi < nmw) a.mask_out[mi] = mw;
+        if (PRED && a.mask_out && gg) {
+            const uint32_t w = p >> 6, off = p & 63;
+            atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
+            if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
         }
     }
     __syncthreads();
+    if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
+        uint64_t m = s_mask[tid];
+        if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
+        a.mask_out[mw0 + tid] = m;
+    }
     const uint32_t NC = 4 * R;
     const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
-    const uint32_t cinc = wave_incl_scan(cc, lane);
-    const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
+    const uint32_t cinc = wave_incl_scan(cc);
+    uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
+    if (ctile > CW_LDS / VB) ctile = CW_LDS / VB; // never exceeds the tile; bounds the write loop
     uint32_t otile = 0;
     if (OVF) otile = wave_sum32(lane < NC ? s_covf[lane] : 0);
-    if (OK == K_STRIDED && wave == 0) tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, &s_carry);
-    // stage the selected values, compacted
+    // packed outputs: stage the selected values, compacted
 #pragma unroll
-    for (uint32_t r = 0; r < CW_RMAX; r++) {
+    for (uint32_t r = 0; r < RM; r++) {
         if (r >= R) continue;
         const uint32_t ci = wave * R + r;
         uint32_t pos = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r];
-        const uint64_t gg = g[r], xr = x[r];
-        for (uint32_t s = 0; s < eps; s++)
-            if ((gg >> s) & 1) s_vals[pos++] = (V)((xr >> (s * fw)) & vm);
+        const uint32_t gg = g[r];
+        if (WIDE) {
+            if (gg & 1) s_vals[pos] = (V)(x[r] & vm);
+        }
+        else if (gg) {
+            const uint32_t lo = (uint32_t)x[r], hi = (uint32_t)(x[r] >> 32);
+            for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                if ((gg >> s) & 1) s_vals[pos++] = (V)(field32(lo, hi, sh) & vm32);
+        }
     }
     __syncthreads();
-    if (OK != K_STRIDED) {
-        if (wave == 0) {
-            const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
-            tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
-        }
-        __syncthreads();
+    if (wave == 0) {
+        const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
+        tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
     }
+    __syncthreads();
     tile_write<OK>(a, s_vals, s_P, s_carry, ctile, t == a.T - 1, tid);
+}
+
+// ------------------------------------------------------------ direct kernel
+//
+// Container outputs (1/2/4/8-byte) from slab / container inputs. The input
+// tile (256*R words, R = min(8, 64/eps)) is copied into LDS by the load unit
+// itself (global_load_lds: no registers are held while it is in flight).
+// Each lane then selects the fields of its words (bit-field extracts at
+// wave-uniform offsets), a DPP scan ranks them, one wave looks back, and
+// every lane stores its selected fields at P + rank, re-reading its words
+// from LDS.
+
+// fn(s, v) for the eps fields of a word (lo, hi): field s = bits
+// [s*fw, s*fw + bw) (bw <= fw: value bits within a field or container).
+template <typename Fn>
+__device__ inline void for_fields(uint32_t lo, uint32_t hi, uint32_t eps, uint32_t fw, uint32_t bw, Fn&& fn)
+{
+    if (fw >= 32) { // 32-bit fields or containers (2 per word), or <= 32-bit values in 64-bit containers
+        const uint32_t m = bw >= 32 ? 0xFFFFFFFFu : ((1u << bw) - 1u);
+        fn(0u, lo & m);
+        if (eps > 1) fn(1u, hi & m);
+        return;
+    }
+    uint32_t s = 0, sh = 0;
+    for (; s < eps && sh + fw <= 32; s++, sh += fw) fn(s, __builtin_amdgcn_ubfe(lo, sh, bw));
+    if (s < eps && sh < 32) { // the field across the halves
+        fn(s, __builtin_amdgcn_alignbit(hi, lo, sh) & ((1u << bw) - 1u));
+        s++;
+        sh += fw;
+    }
+    for (; s < eps; s++, sh += fw) fn(s, __builtin_amdgcn_ubfe(hi, sh - 32, bw));
+}
+
+template <typename C, bool OVF, bool WIDE>
+__device__ inline void direct_store(const CompressArgs& a, uint64_t x, uint32_t gg, uint64_t base)
+{
+    C* ob = (C*)a.out.p + base;
+    const uint32_t fw = a.fw, eps = a.eps, bw = a.in.bits;
+    const uint32_t om32 = (uint32_t)a.out.mask;
+    if (WIDE) {
+        if (base < a.cap) *ob = (C)(x & a.in.mask & a.out.mask);
+        return;
+    }
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    uint32_t k = 0;
+    if (base + (uint32_t)__popc(gg) <= a.cap) {
+        for_fields(lo, hi, eps, fw, bw, [&](uint32_t s, uint32_t v) {
+            if ((gg >> s) & 1) {
+                ob[k] = (C)(OVF ? (v & om32) : v);
+                k++;
+            }
+        });
+    }
+    else {
+        for_fields(lo, hi, eps, fw, bw, [&](uint32_t s, uint32_t v) {
+            if ((gg >> s) & 1) {
+                if (base + k < a.cap) ob[k] = (C)(OVF ? (v & om32) : v);
+                k++;
+            }
+        });
+    }
+}
+
+template <bool PRED, bool OVF, bool WIDE>
+__global__ __launch_bounds__(CS_THREADS) void k_compress_direct(CompressArgs a)
+{
+    constexpr uint32_t F = OVF ? LB_OVF : 0;
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    __shared__ uint64_t s_x[CD_WORDS];
+    __shared__ uint64_t s_mask[CW_MASKW];
+    __shared__ uint32_t s_ccnt[4 * CD_RMAX], s_covf[4 * CD_RMAX];
+    __shared__ uint64_t s_P, s_carry;
+    __shared__ uint32_t s_epoch, s_abort;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t t = blockIdx.x;
+    const uint32_t fw = a.fw, eps = a.eps, R = a.R, bw = a.in.bits;
+    const uint64_t n = a.in.n, nw = a.in_words;
+    const uint64_t w0 = t * (uint64_t)(CS_THREADS * R); // first word of the tile
+    const uint64_t nmw = (n + 63) >> 6;
+    const uint32_t NMW = 4 * R * eps;                    // mask words per tile (<= CW_MASKW)
+    const uint64_t mw0 = (w0 * eps) >> 6;
+
+    // ---- the tile's words -> LDS (wave w: local words [64Rw, 64R(w+1)))
+    const uint32_t wl0 = wave * 64 * R, wend = wl0 + 64 * R;
+    if (a.lds16) {
+        for (uint32_t j = 0; j < R; j += 2) { // 128 words per instruction, 16 bytes per lane
+            const uint32_t wl = wl0 + j * 64 + 2 * lane;
+            const uint64_t gw = w0 + wl;
+            if (wl < wend) {
+                if (gw + 2 <= nw) __builtin_amdgcn_global_load_lds(a.in.p + gw, (lds_ptr)(s_x + wl0 + j * 64), 16, 0, 0);
+                else if (gw < nw) s_x[wl] = a.in.p[gw];
+            }
+        }
+    }
+    else {
+        const uint32_t* src = (const uint32_t*)a.in.p;
+        for (uint32_t j = 0; j < 2 * R; j++) { // 32 words per instruction, 4 bytes per lane
+            const uint32_t dl = 2 * wl0 + j * 64 + lane;
+            if (2 * w0 + dl < 2 * nw)
+                __builtin_amdgcn_global_load_lds(src + 2 * w0 + dl, (lds_ptr)((uint32_t*)s_x + 2 * wl0 + j * 64), 4, 0, 0);
+        }
+    }
+    uint64_t mval = 0;
+    if (!PRED && tid < NMW && mw0 + tid < nmw) mval = a.mask_in[mw0 + tid];
+    if (tid == 0) {
+        s_epoch = ld32(&a.hdr->epoch);
+        s_abort = ld32(&a.hdr->abort);
+    }
+    if (tid < NMW) s_mask[tid] = PRED ? 0 : mval;
+    __builtin_amdgcn_s_waitcnt(0); // the global->LDS copies of this wave
+    __syncthreads();
+    if (s_abort) {
+        if (t == a.T - 1 && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
+        return;
+    }
+    const Lookback<F> L = make_lookback<F>(a, s_epoch);
+    L.clear_slice(a.T, t * CS_THREADS + tid, (uint64_t)a.T * CS_THREADS);
+
+    const uint32_t om32 = (uint32_t)a.out.mask;
+    uint32_t g[CD_RMAX], lp[CD_RMAX];
+#pragma unroll
+    for (uint32_t r = 0; r < CD_RMAX; r++) {
+        g[r] = 0;
+        lp[r] = 0;
+        if (r >= R) continue;
+        const uint32_t ci = wave * R + r;
+        const uint32_t wl = ci * 64 + lane;
+        const uint64_t x = s_x[wl];
+        const uint64_t e0 = (w0 + wl) * eps;
+        const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
+        const uint32_t p = wl * eps; // first bit of the word's elements in the tile mask
+        uint32_t gg = 0, go = 0;
+        if (!PRED) {
+            const uint32_t w = p >> 6, off = p & 63;
+            uint64_t m = s_mask[w] >> off;
+            if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
+            gg = (uint32_t)m;
+        }
+        if (WIDE) {
+            const uint64_t v = x & a.in.mask;
+            if (PRED) gg = a.pr(v);
+            if (OVF) go = (gg & 1) && (v & ~a.out.mask) != 0;
+        }
+        else if (PRED) {
+            for_fields((uint32_t)x, (uint32_t)(x >> 32), eps, fw, bw, [&](uint32_t s, uint32_t v) {
+                const bool sel = a.pr.sel32(v);
+                gg |= (uint32_t)sel << s;
+                if (OVF) go += sel && (v & ~om32) != 0;
+            });
+        }
+        else if (OVF) {
+            for_fields((uint32_t)x, (uint32_t)(x >> 32), eps, fw, bw, [&](uint32_t s, uint32_t v) {
+                go += ((gg >> s) & 1) && (v & ~om32) != 0;
+            });
+        }
+        gg &= (uint32_t)lowmask(nv);
+        if (OVF && nv < eps) { // recount within the valid fields
+            go = 0;
+            if (WIDE) go = (gg & 1) && ((x & a.in.mask) & ~a.out.mask) != 0;
+            else
+                for_fields((uint32_t)x, (uint32_t)(x >> 32), eps, fw, bw, [&](uint32_t s, uint32_t v) {
+                    go += ((gg >> s) & 1) && (v & ~om32) != 0;
+                });
+        }
+        g[r] = gg;
+        const uint32_t c = (uint32_t)__popc(gg);
+        const uint32_t inc = wave_incl_scan(c);
+        lp[r] = inc - c;
+        if (lane == 63) s_ccnt[ci] = inc;
+        if (OVF) {
+            const uint32_t oc = wave_sum32(go);
+            if (lane == 0) s_covf[ci] = oc;
+        }
+        if (PRED && a.mask_out && gg) {
+            const uint32_t w = p >> 6, off = p & 63;
+            atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
+            if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
+        }
+    }
+    __syncthreads();
+    if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
+        uint64_t m = s_mask[tid];
+        if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
+        a.mask_out[mw0 + tid] = m;
+    }
+    const uint32_t NC = 4 * R;
+    const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
+    const uint32_t cinc = wave_incl_scan(cc);
+    const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
+    uint32_t otile = 0;
+    if (OVF) otile = wave_sum32(lane < NC ? s_covf[lane] : 0);
+    if (wave == 0) tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, &s_carry);
+    __syncthreads();
+    const uint64_t P = s_P;
+#pragma unroll
+    for (uint32_t r = 0; r < CD_RMAX; r++) {
+        if (r >= R || !g[r]) continue;
+        const uint32_t ci = wave * R + r;
+        const uint64_t x = s_x[ci * 64 + lane];
+        const uint64_t base = P + (__builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci)) + lp[r];
+        switch (a.out.cbytes) {
+            case 1: direct_store<uint8_t, OVF, WIDE>(a, x, g[r], base); break;
+            case 2: direct_store<uint16_t, OVF, WIDE>(a, x, g[r], base); break;
+            case 4: direct_store<uint32_t, OVF, WIDE>(a, x, g[r], base); break;
+            default: direct_store<uint64_t, OVF, WIDE>(a, x, g[r], base);
+        }
+    }
+    if (t == a.T - 1 && tid == 0) {
+        uint64_t K = P + ctile;
+        if (K > a.cap) K = a.cap;
+        const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
+        for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
+    }
 }
 
 // ------------------------------------------------------------ element kernel
@@ -474,7 +716,11 @@ template <int OK, bool PRED, bool OVF>
 static void launch_words(const CompressArgs& a, uint32_t vb, bool wide, hipStream_t s)
 {
     const dim3 grid(a.T), block(CS_THREADS);
-    if (wide) {
+    if constexpr (OK == K_STRIDED) {
+        if (wide) hipLaunchKernelGGL((k_compress_direct<PRED, OVF, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((k_compress_direct<PRED, OVF, false>), grid, block, 0, s, a);
+    }
+    else if (wide) {
         switch (vb) {
             case 1: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 1, true>), grid, block, 0, s, a); break;
             case 2: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 2, true>), grid, block, 0, s, a); break;
@@ -563,12 +809,20 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     a.mask_out = mask_out;
     a.stats = stats_dev;
     a.hdr = (WsHeader*)ws;
-    const bool words = ci.kind != K_BITSTREAM;
+    if (ci.kind == K_SLAB && ci.bits == 1) ci.kind = K_BITSTREAM; // 64 one-bit fields per word = dense stream
+    a.in = ci;
+    const bool words = ci.kind != K_BITSTREAM; // word kernel: at most 32 fields per word
     uint64_t T;
     if (words) {
         a.fw = ci.kind == K_SLAB ? ci.bits : 8 * ci.cbytes;
         a.eps = 64 / a.fw;
-        a.R = a.eps <= 8 ? 4 : (a.eps <= 16 ? 2 : 1);
+        if (co.kind == K_STRIDED) { // mask words per tile 4*R*eps <= 256
+            a.R = 64 / a.eps;
+            if (a.R > CD_RMAX) a.R = CD_RMAX;
+            a.lds16 = ((uintptr_t)ci.p & 15) == 0;
+        }
+        else // staged values 256*R*eps*VB <= 16 KB
+            a.R = a.eps <= 8 ? 4 : (a.eps <= 16 ? 2 : 1);
         a.in_words = (ci.n + a.eps - 1) / a.eps;
         const uint64_t TW = (uint64_t)CS_THREADS * a.R;
         T = a.in_words ? (a.in_words + TW - 1) / TW : 1;
