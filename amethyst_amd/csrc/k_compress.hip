@@ -37,8 +37,6 @@ namespace am {
 
 constexpr uint32_t CS_THREADS = 256;          // 4 waves
 constexpr uint32_t CW_RMAX = 4;               // word kernel (packed outputs): 64-word chunks per wave (max)
-constexpr uint32_t CD_RMAX = 8;               // direct kernel (container outputs): 64-word chunks per wave (max)
-constexpr uint32_t CD_WORDS = 4 * 64 * CD_RMAX; // direct kernel: input tile words staged in LDS
 constexpr uint32_t CW_LDS = 16384;            // word kernel: bytes of staged values
 constexpr uint32_t CE_J = 16;                 // element kernel: elements per lane
 constexpr uint32_t CE_TILE = 4 * CE_J * 64;   // element kernel: 4096 elements per tile
@@ -64,7 +62,9 @@ struct CompressArgs {
     uint64_t* lb_tiles;
     uint64_t* lb_groups;
     uint64_t lb_cap;
-    uint32_t lds16; // input pointer 16-byte aligned: 16-byte global->LDS copies
+    uint32_t lds16;   // input pointer 16-byte aligned: 16-byte global->LDS copies
+    uint32_t out_vec; // output pointer aligned for 4-element vector stores
+    uint32_t lo_top, rg_top, inv_bits; // predicate scaled to top-aligned fields (direct kernel)
 };
 
 __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
@@ -138,7 +138,7 @@ __device__ inline void tile_lookback(const CompressArgs& a, const Lookback<F>& L
     const Agg ex = L.run(t, own, lane);
     if (lane == 0) {
         *s_P = ex.c;
-        *s_carry = ex.w;
+        if (s_carry) *s_carry = ex.w;
         if (t == a.T - 1) {
             const bool bad = L.aborted();
             if (a.stats) {
@@ -404,211 +404,239 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
 
 // ------------------------------------------------------------ direct kernel
 //
-// Container outputs (1/2/4/8-byte) from slab / container inputs. The input
-// tile (256*R words, R = min(8, 64/eps)) is copied into LDS by the load unit
-// itself (global_load_lds: no registers are held while it is in flight).
-// Each lane then selects the fields of its words (bit-field extracts at
-// wave-uniform offsets), a DPP scan ranks them, one wave looks back, and
-// every lane stores its selected fields at P + rank, re-reading its words
-// from LDS.
+// Container outputs (1/2/4/8-byte) from slab / container inputs. A block is
+// 4 data waves and one look-back wave. The input tile (256*R words,
+// R <= 8) is copied into LDS by the load unit itself (global_load_lds: no
+// registers are held while it is in flight); each data wave then takes its
+// words into registers and forms their selection bits, a DPP scan ranks
+// them, and -- while the look-back wave waits on the predecessors -- the
+// data waves compact the selected values into the same LDS (now free) at
+// the narrowest width that holds them. Once the tile's output offset P is
+// known, all waves copy the compacted run to the output with 16-byte stores.
+constexpr uint32_t CD_RMAX = 8;
+constexpr uint32_t CD_WAVES = 4;                        // data waves; wave CD_WAVES looks back
+constexpr uint32_t CD_THREADS = (CD_WAVES + 1) * 64;
+constexpr uint32_t CD_WORDS = CD_WAVES * 64 * CD_RMAX;  // input words per tile (max)
+constexpr uint32_t CD_STAGE = 20480;                    // LDS bytes: input words, then compacted values
 
-// fn(s, v) for the eps fields of a word (lo, hi): field s = bits
-// [s*fw, s*fw + bw) (bw <= fw: value bits within a field or container).
-template <typename Fn>
-__device__ inline void for_fields(uint32_t lo, uint32_t hi, uint32_t eps, uint32_t fw, uint32_t bw, Fn&& fn)
+// Selection bits of a word's eps fields (field s -> bit s; field s is bits
+// [s*fw, s*fw + bw)). Shifting the word left by 64 - s*fw - bw puts field s
+// at the top of the high half with lower fields' bits beneath it; the range
+// test is scaled by 2^(32-bw) so those bits cannot change its outcome:
+//   v in [lo, lo + span]  <=>  (y - lo*2^k) <= (span + 1)*2^k - 1,  k = 32 - bw.
+// Compare and accumulate are one v_cmp + one v_addc (carry = the compare).
+__device__ inline uint32_t select_fields(uint64_t x, uint32_t eps, uint32_t fw, uint32_t bw, uint32_t LO, uint32_t RG)
 {
-    if (fw >= 32) { // 32-bit fields or containers (2 per word), or <= 32-bit values in 64-bit containers
-        const uint32_t m = bw >= 32 ? 0xFFFFFFFFu : ((1u << bw) - 1u);
-        fn(0u, lo & m);
-        if (eps > 1) fn(1u, hi & m);
-        return;
+    uint32_t gr = 0; // bits in reverse field order
+    for (uint32_t s = 0, sh = 64 - bw; s < eps; s++, sh -= fw) {
+        const uint32_t t = (uint32_t)((x << sh) >> 32) - LO;
+        uint32_t ng;
+        asm("v_cmp_ge_u32 vcc, %2, %1\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc" : "=v"(ng) : "v"(t), "s"(RG), "v"(gr) : "vcc");
+        gr = ng;
     }
-    uint32_t s = 0, sh = 0;
-    for (; s < eps && sh + fw <= 32; s++, sh += fw) fn(s, __builtin_amdgcn_ubfe(lo, sh, bw));
-    if (s < eps && sh < 32) { // the field across the halves
-        fn(s, __builtin_amdgcn_alignbit(hi, lo, sh) & ((1u << bw) - 1u));
-        s++;
-        sh += fw;
-    }
-    for (; s < eps; s++, sh += fw) fn(s, __builtin_amdgcn_ubfe(hi, sh - 32, bw));
+    return __builtin_bitreverse32(gr) >> (32 - eps);
 }
 
-template <typename C, bool OVF, bool WIDE>
-__device__ inline void direct_store(const CompressArgs& a, uint64_t x, uint32_t gg, uint64_t base)
+// Copy values s_vals[0, cnt) to out[P, P + cnt) (capacity-checked), four
+// per thread step with one vector store where the output is aligned.
+template <typename C, typename V>
+__device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t P, uint32_t cnt, uint32_t tid)
 {
-    C* ob = (C*)a.out.p + base;
-    const uint32_t fw = a.fw, eps = a.eps, bw = a.in.bits;
-    const uint32_t om32 = (uint32_t)a.out.mask;
-    if (WIDE) {
-        if (base < a.cap) *ob = (C)(x & a.in.mask & a.out.mask);
-        return;
-    }
-    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    uint32_t k = 0;
-    if (base + (uint32_t)__popc(gg) <= a.cap) {
-        for_fields(lo, hi, eps, fw, bw, [&](uint32_t s, uint32_t v) {
-            if ((gg >> s) & 1) {
-                ob[k] = (C)(OVF ? (v & om32) : v);
-                k++;
+    C* out = (C*)a.out.p;
+    const uint64_t cap = a.cap;
+    uint32_t head = (uint32_t)((4 - (P & 3)) & 3);
+    if (!a.out_vec) head = cnt; // unaligned output buffer: element stores only
+    if (head > cnt) head = cnt;
+    for (uint32_t i = tid; i < head; i += CD_THREADS)
+        if (P + i < cap) out[P + i] = (C)s_vals[i];
+    const uint32_t nq = (cnt - head) / 4;
+    for (uint32_t q = tid; q < nq; q += CD_THREADS) {
+        const uint32_t i0 = head + 4 * q;
+        const uint64_t r = P + i0;
+        const C v0 = (C)s_vals[i0], v1 = (C)s_vals[i0 + 1], v2 = (C)s_vals[i0 + 2], v3 = (C)s_vals[i0 + 3];
+        if (r + 4 <= cap) {
+            if constexpr (sizeof(C) == 4) *(uint4*)(out + r) = make_uint4(v0, v1, v2, v3);
+            else if constexpr (sizeof(C) == 2)
+                *(uint2*)(out + r) = make_uint2((uint32_t)v0 | ((uint32_t)v1 << 16), (uint32_t)v2 | ((uint32_t)v3 << 16));
+            else if constexpr (sizeof(C) == 1)
+                *(uint32_t*)(out + r) = (uint32_t)v0 | ((uint32_t)v1 << 8) | ((uint32_t)v2 << 16) | ((uint32_t)v3 << 24);
+            else {
+                *(ulonglong2*)(out + r) = make_ulonglong2(v0, v1);
+                *(ulonglong2*)(out + r + 2) = make_ulonglong2(v2, v3);
             }
-        });
+        }
+        else {
+            if (r < cap) out[r] = v0;
+            if (r + 1 < cap) out[r + 1] = v1;
+            if (r + 2 < cap) out[r + 2] = v2;
+            if (r + 3 < cap) out[r + 3] = v3;
+        }
     }
-    else {
-        for_fields(lo, hi, eps, fw, bw, [&](uint32_t s, uint32_t v) {
-            if ((gg >> s) & 1) {
-                if (base + k < a.cap) ob[k] = (C)(OVF ? (v & om32) : v);
-                k++;
-            }
-        });
-    }
+    for (uint32_t i = head + 4 * nq + tid; i < cnt; i += CD_THREADS)
+        if (P + i < cap) out[P + i] = (C)s_vals[i];
 }
 
-template <bool PRED, bool OVF, bool WIDE>
-__global__ __launch_bounds__(CS_THREADS) void k_compress_direct(CompressArgs a)
+template <bool PRED, bool OVF, int VB, bool WIDE>
+__global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
 {
     constexpr uint32_t F = OVF ? LB_OVF : 0;
+    using V = typename VT<VB>::T;
     typedef __attribute__((address_space(3))) void* lds_ptr;
-    __shared__ uint64_t s_x[CD_WORDS];
+    __shared__ uint64_t s_buf[CD_STAGE / 8]; // input words, then compacted values
     __shared__ uint64_t s_mask[CW_MASKW];
     __shared__ uint32_t s_ccnt[4 * CD_RMAX], s_covf[4 * CD_RMAX];
-    __shared__ uint64_t s_P, s_carry;
+    __shared__ uint64_t s_P;
     __shared__ uint32_t s_epoch, s_abort;
+    uint64_t* const s_x = s_buf;
+    V* const s_vals = (V*)s_buf;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool ctrl = wave == CD_WAVES;
     const uint64_t t = blockIdx.x;
     const uint32_t fw = a.fw, eps = a.eps, R = a.R, bw = a.in.bits;
     const uint64_t n = a.in.n, nw = a.in_words;
-    const uint64_t w0 = t * (uint64_t)(CS_THREADS * R); // first word of the tile
+    const uint64_t w0 = t * (uint64_t)(CD_WAVES * 64 * R); // first word of the tile
     const uint64_t nmw = (n + 63) >> 6;
-    const uint32_t NMW = 4 * R * eps;                    // mask words per tile (<= CW_MASKW)
+    const uint32_t NMW = CD_WAVES * R * eps;                 // mask words per tile (<= CW_MASKW)
     const uint64_t mw0 = (w0 * eps) >> 6;
+    const bool last = t == a.T - 1;
 
-    // ---- the tile's words -> LDS (wave w: local words [64Rw, 64R(w+1)))
-    const uint32_t wl0 = wave * 64 * R, wend = wl0 + 64 * R;
-    if (a.lds16) {
-        for (uint32_t j = 0; j < R; j += 2) { // 128 words per instruction, 16 bytes per lane
-            const uint32_t wl = wl0 + j * 64 + 2 * lane;
-            const uint64_t gw = w0 + wl;
-            if (wl < wend) {
-                if (gw + 2 <= nw) __builtin_amdgcn_global_load_lds(a.in.p + gw, (lds_ptr)(s_x + wl0 + j * 64), 16, 0, 0);
-                else if (gw < nw) s_x[wl] = a.in.p[gw];
+    // ---- loads: the tile's words -> LDS (data wave w: local words [64Rw, 64R(w+1)))
+    uint32_t ep = 0, ab = 0;
+    if (!ctrl) {
+        const uint32_t wl0 = wave * 64 * R, wend = wl0 + 64 * R;
+        if (a.lds16) {
+            for (uint32_t j = 0; j < R; j += 2) { // 128 words per instruction, 16 bytes per lane
+                const uint32_t wl = wl0 + j * 64 + 2 * lane;
+                const uint64_t gw = w0 + wl;
+                if (wl < wend) {
+                    if (gw + 2 <= nw) __builtin_amdgcn_global_load_lds(a.in.p + gw, (lds_ptr)(s_x + wl0 + j * 64), 16, 0, 0);
+                    else if (gw < nw) s_x[wl] = a.in.p[gw];
+                }
+            }
+        }
+        else {
+            const uint32_t* src = (const uint32_t*)a.in.p;
+            for (uint32_t j = 0; j < 2 * R; j++) { // 32 words per instruction, 4 bytes per lane
+                const uint32_t dl = 2 * wl0 + j * 64 + lane;
+                if (2 * w0 + dl < 2 * nw)
+                    __builtin_amdgcn_global_load_lds(src + 2 * w0 + dl, (lds_ptr)((uint32_t*)s_x + 2 * wl0 + j * 64), 4, 0, 0);
+            }
+        }
+        if (!PRED) {
+            uint64_t m = 0;
+            const uint32_t i = tid; // NMW <= 256 = data threads
+            if (i < NMW && mw0 + i < nmw) m = a.mask_in[mw0 + i];
+            if (i < NMW) s_mask[i] = m;
+        }
+        else if (a.mask_out && tid < NMW)
+            s_mask[tid] = 0;
+        __builtin_amdgcn_s_waitcnt(0); // this wave's global->LDS copies
+    }
+    else {
+        ep = ld32(&a.hdr->epoch); // in flight across the first barrier
+        ab = ld32(&a.hdr->abort);
+    }
+    __syncthreads();
+
+    // ---- data waves: words -> registers, selection bits, ranks
+    uint64_t x[CD_RMAX];
+    uint32_t g[CD_RMAX], lp[CD_RMAX];
+    const uint32_t om32 = (uint32_t)a.out.mask;
+    if (!ctrl) {
+#pragma unroll
+        for (uint32_t r = 0; r < CD_RMAX; r++) {
+            x[r] = 0;
+            g[r] = 0;
+            lp[r] = 0;
+            if (r >= R) continue;
+            const uint32_t ci = wave * R + r;
+            const uint32_t wl = ci * 64 + lane;
+            x[r] = s_x[wl];
+            const uint32_t p = wl * eps; // first bit of the word's elements in the tile mask
+            uint32_t gg = 0, go = 0;
+            if (!PRED) {
+                const uint32_t w = p >> 6, off = p & 63;
+                uint64_t m = s_mask[w] >> off;
+                if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
+                gg = (uint32_t)m & (uint32_t)lowmask(eps);
+            }
+            else if (WIDE)
+                gg = a.pr(x[r] & a.in.mask);
+            else
+                gg = select_fields(x[r], eps, fw, bw, a.lo_top, a.rg_top) ^ a.inv_bits;
+            if (last) { // elements past n (only in the last tile)
+                const uint64_t e0 = (w0 + wl) * eps;
+                const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
+                gg &= (uint32_t)lowmask(nv);
+            }
+            if (OVF) {
+                if (WIDE) go = (gg & 1) && ((x[r] & a.in.mask) & ~a.out.mask) != 0;
+                else
+                    for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                        go += ((gg >> s) & 1) && (((uint32_t)(x[r] >> sh) & (uint32_t)a.in.mask) & ~om32) != 0;
+                const uint32_t oc = wave_sum32(go);
+                if (lane == 0) s_covf[ci] = oc;
+            }
+            g[r] = gg;
+            const uint32_t c = (uint32_t)__popc(gg);
+            const uint32_t inc = wave_incl_scan(c);
+            lp[r] = inc - c;
+            if (lane == 63) s_ccnt[ci] = inc;
+            if (PRED && a.mask_out && gg) {
+                const uint32_t w = p >> 6, off = p & 63;
+                atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
+                if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
             }
         }
     }
-    else {
-        const uint32_t* src = (const uint32_t*)a.in.p;
-        for (uint32_t j = 0; j < 2 * R; j++) { // 32 words per instruction, 4 bytes per lane
-            const uint32_t dl = 2 * wl0 + j * 64 + lane;
-            if (2 * w0 + dl < 2 * nw)
-                __builtin_amdgcn_global_load_lds(src + 2 * w0 + dl, (lds_ptr)((uint32_t*)s_x + 2 * wl0 + j * 64), 4, 0, 0);
-        }
+    else if (lane == 0) {
+        s_epoch = ep;
+        s_abort = ab;
     }
-    uint64_t mval = 0;
-    if (!PRED && tid < NMW && mw0 + tid < nmw) mval = a.mask_in[mw0 + tid];
-    if (tid == 0) {
-        s_epoch = ld32(&a.hdr->epoch);
-        s_abort = ld32(&a.hdr->abort);
-    }
-    if (tid < NMW) s_mask[tid] = PRED ? 0 : mval;
-    __builtin_amdgcn_s_waitcnt(0); // the global->LDS copies of this wave
-    __syncthreads();
+    __syncthreads(); // every word is in registers: s_buf is free for the compacted values
     if (s_abort) {
-        if (t == a.T - 1 && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
+        if (last && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
         return;
     }
-    const Lookback<F> L = make_lookback<F>(a, s_epoch);
-    L.clear_slice(a.T, t * CS_THREADS + tid, (uint64_t)a.T * CS_THREADS);
-
-    const uint32_t om32 = (uint32_t)a.out.mask;
-    uint32_t g[CD_RMAX], lp[CD_RMAX];
-#pragma unroll
-    for (uint32_t r = 0; r < CD_RMAX; r++) {
-        g[r] = 0;
-        lp[r] = 0;
-        if (r >= R) continue;
-        const uint32_t ci = wave * R + r;
-        const uint32_t wl = ci * 64 + lane;
-        const uint64_t x = s_x[wl];
-        const uint64_t e0 = (w0 + wl) * eps;
-        const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
-        const uint32_t p = wl * eps; // first bit of the word's elements in the tile mask
-        uint32_t gg = 0, go = 0;
-        if (!PRED) {
-            const uint32_t w = p >> 6, off = p & 63;
-            uint64_t m = s_mask[w] >> off;
-            if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
-            gg = (uint32_t)m;
-        }
-        if (WIDE) {
-            const uint64_t v = x & a.in.mask;
-            if (PRED) gg = a.pr(v);
-            if (OVF) go = (gg & 1) && (v & ~a.out.mask) != 0;
-        }
-        else if (PRED) {
-            for_fields((uint32_t)x, (uint32_t)(x >> 32), eps, fw, bw, [&](uint32_t s, uint32_t v) {
-                const bool sel = a.pr.sel32(v);
-                gg |= (uint32_t)sel << s;
-                if (OVF) go += sel && (v & ~om32) != 0;
-            });
-        }
-        else if (OVF) {
-            for_fields((uint32_t)x, (uint32_t)(x >> 32), eps, fw, bw, [&](uint32_t s, uint32_t v) {
-                go += ((gg >> s) & 1) && (v & ~om32) != 0;
-            });
-        }
-        gg &= (uint32_t)lowmask(nv);
-        if (OVF && nv < eps) { // recount within the valid fields
-            go = 0;
-            if (WIDE) go = (gg & 1) && ((x & a.in.mask) & ~a.out.mask) != 0;
-            else
-                for_fields((uint32_t)x, (uint32_t)(x >> 32), eps, fw, bw, [&](uint32_t s, uint32_t v) {
-                    go += ((gg >> s) & 1) && (v & ~om32) != 0;
-                });
-        }
-        g[r] = gg;
-        const uint32_t c = (uint32_t)__popc(gg);
-        const uint32_t inc = wave_incl_scan(c);
-        lp[r] = inc - c;
-        if (lane == 63) s_ccnt[ci] = inc;
-        if (OVF) {
-            const uint32_t oc = wave_sum32(go);
-            if (lane == 0) s_covf[ci] = oc;
-        }
-        if (PRED && a.mask_out && gg) {
-            const uint32_t w = p >> 6, off = p & 63;
-            atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
-            if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
-        }
-    }
-    __syncthreads();
-    if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
-        uint64_t m = s_mask[tid];
-        if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
-        a.mask_out[mw0 + tid] = m;
-    }
-    const uint32_t NC = 4 * R;
+    const uint32_t NC = CD_WAVES * R;
     const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
     const uint32_t cinc = wave_incl_scan(cc);
     const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
-    uint32_t otile = 0;
-    if (OVF) otile = wave_sum32(lane < NC ? s_covf[lane] : 0);
-    if (wave == 0) tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, &s_carry);
-    __syncthreads();
-    const uint64_t P = s_P;
+    if (ctrl) {
+        const Lookback<F> L = make_lookback<F>(a, s_epoch);
+        L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
+        const uint32_t otile = OVF ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
+        tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, nullptr);
+    }
+    else {
+        if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
+            uint64_t m = s_mask[tid];
+            if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
+            a.mask_out[mw0 + tid] = m;
+        }
+        const uint64_t vm = a.in.mask & a.out.mask;
 #pragma unroll
-    for (uint32_t r = 0; r < CD_RMAX; r++) {
-        if (r >= R || !g[r]) continue;
-        const uint32_t ci = wave * R + r;
-        const uint64_t x = s_x[ci * 64 + lane];
-        const uint64_t base = P + (__builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci)) + lp[r];
-        switch (a.out.cbytes) {
-            case 1: direct_store<uint8_t, OVF, WIDE>(a, x, g[r], base); break;
-            case 2: direct_store<uint16_t, OVF, WIDE>(a, x, g[r], base); break;
-            case 4: direct_store<uint32_t, OVF, WIDE>(a, x, g[r], base); break;
-            default: direct_store<uint64_t, OVF, WIDE>(a, x, g[r], base);
+        for (uint32_t r = 0; r < CD_RMAX; r++) {
+            if (r >= R || !g[r]) continue;
+            const uint32_t ci = wave * R + r;
+            uint32_t pos = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r];
+            const uint32_t gg = g[r];
+            if (WIDE) s_vals[pos] = (V)(x[r] & vm);
+            else
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                    if ((gg >> s) & 1) s_vals[pos++] = (V)((uint32_t)(x[r] >> sh) & (uint32_t)vm);
         }
     }
-    if (t == a.T - 1 && tid == 0) {
+    __syncthreads();
+    const uint64_t P = s_P;
+    switch (a.out.cbytes) {
+        case 1: copy_out<uint8_t>(a, s_vals, P, ctile, tid); break;
+        case 2: copy_out<uint16_t>(a, s_vals, P, ctile, tid); break;
+        case 4: copy_out<uint32_t>(a, s_vals, P, ctile, tid); break;
+        default: copy_out<uint64_t>(a, s_vals, P, ctile, tid);
+    }
+    if (last && tid == 0) {
         uint64_t K = P + ctile;
         if (K > a.cap) K = a.cap;
         const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
@@ -717,8 +745,22 @@ static void launch_words(const CompressArgs& a, uint32_t vb, bool wide, hipStrea
 {
     const dim3 grid(a.T), block(CS_THREADS);
     if constexpr (OK == K_STRIDED) {
-        if (wide) hipLaunchKernelGGL((k_compress_direct<PRED, OVF, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_compress_direct<PRED, OVF, false>), grid, block, 0, s, a);
+        const dim3 dblock(CD_THREADS);
+        if (wide) {
+            switch (vb) {
+                case 1: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 1, true>), grid, dblock, 0, s, a); break;
+                case 2: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 2, true>), grid, dblock, 0, s, a); break;
+                case 4: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 4, true>), grid, dblock, 0, s, a); break;
+                default: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 8, true>), grid, dblock, 0, s, a);
+            }
+        }
+        else {
+            switch (vb) {
+                case 1: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 1, false>), grid, dblock, 0, s, a); break;
+                case 2: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 2, false>), grid, dblock, 0, s, a); break;
+                default: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 4, false>), grid, dblock, 0, s, a);
+            }
+        }
     }
     else if (wide) {
         switch (vb) {
@@ -816,15 +858,29 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     if (words) {
         a.fw = ci.kind == K_SLAB ? ci.bits : 8 * ci.cbytes;
         a.eps = 64 / a.fw;
-        if (co.kind == K_STRIDED) { // mask words per tile 4*R*eps <= 256
+        if (co.kind == K_STRIDED) {
+            // mask words per tile 4*R*eps <= 256; compacted values 256*R*eps*VB <= CD_STAGE
+            const uint32_t vbh = value_bytes(ci.bits < co.bits ? ci.bits : co.bits);
             a.R = 64 / a.eps;
+            const uint32_t rs = CD_STAGE / (CD_WAVES * 64 * a.eps * vbh);
+            if (a.R > rs) a.R = rs;
             if (a.R > CD_RMAX) a.R = CD_RMAX;
+            if (a.R < 1) a.R = 1;
             a.lds16 = ((uintptr_t)ci.p & 15) == 0;
+            a.out_vec = ((uintptr_t)co.p & (4 * co.cbytes - 1)) == 0;
+            if (pred && !mask && ci.bits <= 32) {
+                // (v - lo) <= span over top-aligned fields: scale by 2^(32 - bits)
+                const uint32_t k = 32 - ci.bits;
+                const uint64_t sp1 = a.pr.span >= ci.mask ? (ci.mask + 1) : a.pr.span + 1;
+                a.lo_top = (uint32_t)(a.pr.lo << k);
+                a.rg_top = (uint32_t)((sp1 << k) - 1);
+                a.inv_bits = a.pr.inv ? (uint32_t)lowmask(a.eps) : 0;
+            }
         }
         else // staged values 256*R*eps*VB <= 16 KB
             a.R = a.eps <= 8 ? 4 : (a.eps <= 16 ? 2 : 1);
         a.in_words = (ci.n + a.eps - 1) / a.eps;
-        const uint64_t TW = (uint64_t)CS_THREADS * a.R;
+        const uint64_t TW = (uint64_t)256 * a.R; // 4 (data) waves x R chunks x 64 words
         T = a.in_words ? (a.in_words + TW - 1) / TW : 1;
     }
     else {
