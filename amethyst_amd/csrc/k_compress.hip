@@ -405,17 +405,18 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
 // ------------------------------------------------------------ direct kernel
 //
 // Container outputs (1/2/4/8-byte) from slab / container inputs. A block is
-// 4 data waves and one look-back wave. The input tile (256*R words,
-// R <= 8) is copied into LDS by the load unit itself (global_load_lds: no
-// registers are held while it is in flight); each data wave then takes its
-// words into registers and forms their selection bits, a DPP scan ranks
-// them, and -- while the look-back wave waits on the predecessors -- the
-// data waves compact the selected values into the same LDS (now free) at
-// the narrowest width that holds them. Once the tile's output offset P is
-// known, all waves copy the compacted run to the output with 16-byte stores.
+// 4 waves (the largest shape of which 7-8 blocks stay resident per CU). The
+// input tile (256*R words, R <= 8) is copied into LDS by the load unit
+// itself (global_load_lds: no registers are held while it is in flight);
+// each wave then takes its words into registers and forms their selection
+// bits, a DPP scan ranks them, and -- while wave 0 looks back over the
+// predecessors -- the other waves compact their selected values into the
+// same LDS (now free) at the narrowest width that holds them; wave 0 does
+// its own share after the look-back. Once the tile's output offset P is
+// known, the block copies the compacted run to the output with vector stores.
 constexpr uint32_t CD_RMAX = 8;
-constexpr uint32_t CD_WAVES = 4;                        // data waves; wave CD_WAVES looks back
-constexpr uint32_t CD_THREADS = (CD_WAVES + 1) * 64;
+constexpr uint32_t CD_WAVES = 4;
+constexpr uint32_t CD_THREADS = CD_WAVES * 64;
 constexpr uint32_t CD_WORDS = CD_WAVES * 64 * CD_RMAX;  // input words per tile (max)
 constexpr uint32_t CD_STAGE = 20480;                    // LDS bytes: input words, then compacted values
 
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     V* const s_vals = (V*)s_buf;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const bool ctrl = wave == CD_WAVES;
+    const bool lead = wave == 0; // also looks back
     const uint64_t t = blockIdx.x;
     const uint32_t fw = a.fw, eps = a.eps, R = a.R, bw = a.in.bits;
     const uint64_t n = a.in.n, nw = a.in_words;
@@ -501,9 +502,13 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     const uint64_t mw0 = (w0 * eps) >> 6;
     const bool last = t == a.T - 1;
 
-    // ---- loads: the tile's words -> LDS (data wave w: local words [64Rw, 64R(w+1)))
+    // ---- loads: the tile's words -> LDS (wave w: local words [64Rw, 64R(w+1)))
     uint32_t ep = 0, ab = 0;
-    if (!ctrl) {
+    if (lead && lane == 0) {
+        ep = ld32(&a.hdr->epoch); // needed after the first barrier
+        ab = ld32(&a.hdr->abort);
+    }
+    {
         const uint32_t wl0 = wave * 64 * R, wend = wl0 + 64 * R;
         if (a.lds16) {
             for (uint32_t j = 0; j < R; j += 2) { // 128 words per instruction, 16 bytes per lane
@@ -525,17 +530,12 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
         }
         if (!PRED) {
             uint64_t m = 0;
-            const uint32_t i = tid; // NMW <= 256 = data threads
-            if (i < NMW && mw0 + i < nmw) m = a.mask_in[mw0 + i];
-            if (i < NMW) s_mask[i] = m;
+            if (tid < NMW && mw0 + tid < nmw) m = a.mask_in[mw0 + tid];
+            if (tid < NMW) s_mask[tid] = m;
         }
         else if (a.mask_out && tid < NMW)
             s_mask[tid] = 0;
         __builtin_amdgcn_s_waitcnt(0); // this wave's global->LDS copies
-    }
-    else {
-        ep = ld32(&a.hdr->epoch); // in flight across the first barrier
-        ab = ld32(&a.hdr->abort);
     }
     __syncthreads();
 
@@ -543,7 +543,8 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     uint64_t x[CD_RMAX];
     uint32_t g[CD_RMAX], lp[CD_RMAX];
     const uint32_t om32 = (uint32_t)a.out.mask;
-    if (!ctrl) {
+    const uint32_t leps = __umul24(lane, eps);
+    {
 #pragma unroll
         for (uint32_t r = 0; r < CD_RMAX; r++) {
             x[r] = 0;
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             const uint32_t ci = wave * R + r;
             const uint32_t wl = ci * 64 + lane;
             x[r] = s_x[wl];
-            const uint32_t p = wl * eps; // first bit of the word's elements in the tile mask
+            const uint32_t p = ci * 64 * eps + leps; // first bit of the word's elements in the tile mask
             uint32_t gg = 0, go = 0;
             if (!PRED) {
                 const uint32_t w = p >> 6, off = p & 63;
@@ -590,7 +591,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             }
         }
     }
-    else if (lane == 0) {
+    if (lead && lane == 0) {
         s_epoch = ep;
         s_abort = ab;
     }
@@ -603,30 +604,29 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
     const uint32_t cinc = wave_incl_scan(cc);
     const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
-    if (ctrl) {
+    if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
+        uint64_t m = s_mask[tid];
+        if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
+        a.mask_out[mw0 + tid] = m;
+    }
+    if (lead) {
         const Lookback<F> L = make_lookback<F>(a, s_epoch);
         L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
         const uint32_t otile = OVF ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
         tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, nullptr);
     }
-    else {
-        if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
-            uint64_t m = s_mask[tid];
-            if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
-            a.mask_out[mw0 + tid] = m;
-        }
-        const uint64_t vm = a.in.mask & a.out.mask;
+    // compact the selected values into LDS at their tile ranks
+    const uint32_t vm32 = (uint32_t)(a.in.mask & a.out.mask);
 #pragma unroll
-        for (uint32_t r = 0; r < CD_RMAX; r++) {
-            if (r >= R || !g[r]) continue;
-            const uint32_t ci = wave * R + r;
-            uint32_t pos = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r];
-            const uint32_t gg = g[r];
-            if (WIDE) s_vals[pos] = (V)(x[r] & vm);
-            else
-                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
-                    if ((gg >> s) & 1) s_vals[pos++] = (V)((uint32_t)(x[r] >> sh) & (uint32_t)vm);
-        }
+    for (uint32_t r = 0; r < CD_RMAX; r++) {
+        if (r >= R || !g[r]) continue;
+        const uint32_t ci = wave * R + r;
+        V* dst = s_vals + (__builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r]);
+        const uint32_t gg = g[r];
+        if (WIDE) *dst = (V)(x[r] & a.in.mask & a.out.mask);
+        else
+            for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                if (gg & (1u << s)) *dst++ = (V)((uint32_t)(x[r] >> sh) & vm32);
     }
     __syncthreads();
     const uint64_t P = s_P;
