@@ -81,8 +81,10 @@ def read_traffic(name):
                 pass
     if "fetch_kib_per_dispatch" not in vals or "write_kib_per_dispatch" not in vals:
         return None
-    scale = vals.get("copy_scale", 1.0)
-    return (vals["fetch_kib_per_dispatch"] + vals["write_kib_per_dispatch"]) * 1024.0 * scale
+    # each counter scaled by what it reads for a plain copy kernel of known size
+    fs = vals.get("fetch_scale", 1.0)
+    ws = vals.get("write_scale", 1.0)
+    return (vals["fetch_kib_per_dispatch"] * fs + vals["write_kib_per_dispatch"] * ws) * 1024.0
 
 
 def cpu_baseline(am, torch, col, pred_c):
@@ -422,7 +424,7 @@ def main():
                    "bytes_per_step": byt},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_compress<K_SLAB,K_STRIDED,true>", "avg_kernel_ms": round(avg_kernel, 5)},
+                     "kernel": "k_compress_direct<PRED=1,OVF=0,VB=2,WIDE=0>", "avg_kernel_ms": round(avg_kernel, 5)},
         "cpu_baseline": cpu,
         "ops": ops,
     }
