@@ -162,6 +162,31 @@ struct Fetch {
     }
 };
 
+// A predicate (values of `bits` <= 32 bits) over fields read top-aligned in a
+// 32-bit register with arbitrary bits beneath: see select_fields.
+struct TopPred {
+    uint32_t lo, rg, inv;
+};
+
+// Selection bits of a word's eps fields (field s -> bit s; field s is bits
+// [s*fw, s*fw + bw)). Shifting the word left by 64 - s*fw - bw puts field s
+// at the top of the high half with lower fields' bits beneath it; the range
+// test is scaled by 2^(32-bw) so those bits cannot change its outcome:
+//   v in [lo, lo + span]  <=>  (y - lo*2^k) <= (span + 1)*2^k - 1,  k = 32 - bw.
+// Compare and accumulate are one v_cmp + one v_addc (carry = the compare).
+__device__ inline uint32_t select_fields(uint64_t x, uint32_t eps, uint32_t fw, uint32_t bw, const TopPred& tp)
+{
+    const uint32_t LO = tp.lo, RG = tp.rg;
+    uint32_t gr = 0; // bits in reverse field order
+    for (uint32_t s = 0, sh = 64 - bw; s < eps; s++, sh -= fw) {
+        const uint32_t t = (uint32_t)((x << sh) >> 32) - LO;
+        uint32_t ng;
+        asm("v_cmp_ge_u32 vcc, %2, %1\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc" : "=v"(ng) : "v"(t), "s"(RG), "v"(gr) : "vcc");
+        gr = ng;
+    }
+    return (__builtin_bitreverse32(gr) >> (32 - eps)) ^ tp.inv;
+}
+
 // ------------------------------------------------------------ wave64 helpers
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
@@ -212,6 +237,27 @@ __device__ inline bool pred_eval(uint32_t op, uint64_t c, uint64_t v)
         case AM_PRED_EQ: return v == c;
         default: return v != c;
     }
+}
+
+// OR over aligned segments of S = 64/eps lanes (S a power of two, 2..32):
+// after the call the last lane of each segment holds the OR of the segment
+// (Hillis-Steele with DPP row_shr 1/2/4/8 inside rows of 16, then row_bcast
+// 15 carries row 0/2 into row 1/3 for S = 32).
+template <int CTRL, int ROWM, bool BC>
+__device__ inline uint64_t dpp_or64(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWM, 0xf, BC);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, ROWM, 0xf, BC);
+    return v | (((uint64_t)hi << 32) | lo);
+}
+__device__ inline uint64_t segment_or(uint64_t v, uint32_t S)
+{
+    if (S >= 2) v = dpp_or64<0x111, 0xf, true>(v);
+    if (S >= 4) v = dpp_or64<0x112, 0xf, true>(v);
+    if (S >= 8) v = dpp_or64<0x114, 0xf, true>(v);
+    if (S >= 16) v = dpp_or64<0x118, 0xf, true>(v);
+    if (S >= 32) v = dpp_or64<0x142, 0xa, false>(v);
+    return v;
 }
 
 // ------------------------------------------------------------ agent-scope atomics

@@ -78,6 +78,21 @@ inline Pred make_pred(const am_predicate& p, uint64_t vmax)
     return r;
 }
 
+// The predicate scaled to top-aligned fields: (v - lo) <= span over values
+// of `bits` <= 32 bits becomes (y - lo*2^k) <= (span+1)*2^k - 1, k = 32 - bits,
+// for y = v*2^k + (anything < 2^k); inv flips the eps selection bits.
+inline TopPred make_top_pred(const Pred& p, uint32_t bits, uint32_t eps)
+{
+    const uint32_t k = 32 - bits;
+    const uint64_t vmask = lowmask(bits);
+    const uint64_t sp1 = p.span >= vmask ? (vmask + 1) : p.span + 1;
+    TopPred t;
+    t.lo = (uint32_t)(p.lo << k);
+    t.rg = (uint32_t)((sp1 << k) - 1);
+    t.inv = p.inv ? (uint32_t)lowmask(eps) : 0;
+    return t;
+}
+
 inline int check_launch()
 {
     hipError_t e = hipGetLastError();

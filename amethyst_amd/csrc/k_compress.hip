@@ -36,8 +36,6 @@
 namespace am {
 
 constexpr uint32_t CS_THREADS = 256;          // 4 waves
-constexpr uint32_t CW_RMAX = 4;               // word kernel (packed outputs): 64-word chunks per wave (max)
-constexpr uint32_t CW_LDS = 16384;            // word kernel: bytes of staged values
 constexpr uint32_t CE_J = 16;                 // element kernel: elements per lane
 constexpr uint32_t CE_TILE = 4 * CE_J * 64;   // element kernel: 4096 elements per tile
 
@@ -64,7 +62,7 @@ struct CompressArgs {
     uint64_t lb_cap;
     uint32_t lds16;   // input pointer 16-byte aligned: 16-byte global->LDS copies
     uint32_t out_vec; // output pointer aligned for 4-element vector stores
-    uint32_t lo_top, rg_top, inv_bits; // predicate scaled to top-aligned fields (direct kernel)
+    TopPred tp;       // predicate over top-aligned fields (direct kernel)
 };
 
 __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
@@ -236,172 +234,6 @@ struct CsFlags {
     static constexpr uint32_t F = OK != K_STRIDED ? (LB_WIN | LB_OVF) : (OVF ? LB_OVF : 0);
 };
 
-// ------------------------------------------------------------ word kernel
-//
-// Packed (slab / bitstream) outputs from slab / container inputs.
-// Tile = 4 waves x R chunks x 64 words (R = 4, 2, 1 for eps <= 8, <= 16,
-// <= 32), i.e. 256*R*eps elements, a multiple of 64, so every tile starts on
-// an output mask word. Lane L of chunk c holds input word c*64+L, i.e. the
-// elements (c*64+L)*eps .. +eps-1 of the tile; its selection bits g go to
-// bit (c*64+L)*eps of the tile's mask (LDS, atomic OR), its rank among the
-// tile's selected values is the chunk's prefix plus a wave scan of the
-// per-lane popcounts (DPP, no LDS round trips).
-constexpr uint32_t CW_MASKW = 256; // output mask words per tile (max: R=1, eps=64)
-
-// field s of an input word (lo, hi halves) at bit offset sh (< 64), 32-bit result
-__device__ inline uint32_t field32(uint32_t lo, uint32_t hi, uint32_t sh)
-{
-    return sh < 32 ? __builtin_amdgcn_alignbit(hi, lo, sh) : (hi >> (sh - 32));
-}
-
-template <int OK, bool PRED, bool OVF, int VB, bool WIDE>
-__global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
-{
-    static_assert(OK != K_STRIDED, "container outputs use k_compress_direct");
-    constexpr uint32_t F = CsFlags<OK, OVF>::F;
-    constexpr uint32_t RM = CW_RMAX; // chunks per wave (max)
-    using V = typename VT<VB>::T;
-    __shared__ V s_vals[CW_LDS / VB];
-    __shared__ uint64_t s_mask[CW_MASKW]; // output mask (mask_out) or input mask (mask_in) of the tile
-    __shared__ uint32_t s_ccnt[4 * RM], s_covf[4 * RM];
-    __shared__ uint64_t s_P, s_carry;
-    __shared__ uint32_t s_epoch, s_abort;
-
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t t = blockIdx.x;
-    const uint32_t fw = a.fw, eps = a.eps, R = a.R;
-    const uint64_t n = a.in.n;
-    const uint64_t TW = (uint64_t)CS_THREADS * R; // words per tile
-    const uint64_t nmw = (n + 63) >> 6;
-    const uint32_t NMW = 4 * R * eps;              // mask words per tile (<= CW_MASKW)
-    const uint64_t mw0 = (t * TW * eps) >> 6;      // first mask word of the tile
-
-    // ---- loads first: input mask words, input words, then the epoch
-    uint64_t mval = 0;
-    if (!PRED && tid < NMW && mw0 + tid < nmw) mval = a.mask_in[mw0 + tid];
-    uint64_t x[RM];
-#pragma unroll
-    for (uint32_t r = 0; r < RM; r++) {
-        x[r] = 0;
-        if (r < R) {
-            const uint64_t wi = t * TW + (uint64_t)(wave * R + r) * 64 + lane;
-            if (wi < a.in_words) x[r] = a.in.p[wi];
-        }
-    }
-    if (tid == 0) {
-        s_epoch = ld32(&a.hdr->epoch);
-        s_abort = ld32(&a.hdr->abort);
-    }
-    if (tid < NMW) s_mask[tid] = PRED ? 0 : mval;
-    __syncthreads();
-    if (s_abort) {
-        if (t == a.T - 1 && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
-        return;
-    }
-    const Lookback<F> L = make_lookback<F>(a, s_epoch);
-    L.clear_slice(a.T, t * CS_THREADS + tid, (uint64_t)a.T * CS_THREADS);
-
-    const uint64_t om = a.out.mask;
-    const uint32_t m32 = (uint32_t)a.in.mask, vm32 = (uint32_t)(a.in.mask & om);
-    const uint64_t vm = a.in.mask & om;
-    uint32_t g[RM];
-    uint32_t lp[RM];
-#pragma unroll
-    for (uint32_t r = 0; r < RM; r++) {
-        g[r] = 0;
-        lp[r] = 0;
-        if (r >= R) continue;
-        const uint32_t ci = wave * R + r;
-        const uint64_t e0 = (t * TW + (uint64_t)ci * 64 + lane) * eps;
-        const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
-        const uint32_t p = (ci * 64 + lane) * eps; // bit of the tile mask
-        const uint32_t lo = (uint32_t)x[r], hi = (uint32_t)(x[r] >> 32);
-        uint32_t gg = 0;
-        uint32_t go = 0;
-        if (PRED) {
-            if (WIDE) {
-                const uint64_t v = x[r] & a.in.mask; // eps == 1
-                gg = a.pr(v);
-                if (OVF) go = gg && (v & ~om) != 0;
-            }
-            else {
-                uint32_t g32 = 0, o32 = 0;
-                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw) {
-                    const uint32_t v = field32(lo, hi, sh) & m32;
-                    const bool sel = a.pr.sel32(v);
-                    g32 |= (uint32_t)sel << s;
-                    if (OVF) o32 += sel && (v & ~(uint32_t)om) != 0;
-                }
-                gg = g32;
-                go = o32;
-            }
-        }
-        else {
-            const uint32_t w = p >> 6, off = p & 63;
-            uint64_t m = s_mask[w] >> off;
-            if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
-            gg = (uint32_t)m;
-            if (OVF) {
-                if (WIDE) go = (gg & 1) && ((x[r] & a.in.mask) & ~om) != 0;
-                else
-                    for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
-                        go += ((gg >> s) & 1) && ((field32(lo, hi, sh) & m32) & ~(uint32_t)om) != 0;
-            }
-        }
-        gg &= (uint32_t)lowmask(nv);
-        g[r] = gg;
-        const uint32_t c = (uint32_t)__popc(gg);
-        const uint32_t inc = wave_incl_scan(c);
-        lp[r] = inc - c;
-        if (lane == 63) s_ccnt[ci] = inc;
-        if (OVF) {
-            const uint32_t oc = wave_sum32(go);
-            if (lane == 0) s_covf[ci] = oc;
-        }
-        if (PRED && a.mask_out && gg) {
-            const uint32_t w = p >> 6, off = p & 63;
-            atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
-            if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
-        }
-    }
-    __syncthreads();
-    if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
-        uint64_t m = s_mask[tid];
-        if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
-        a.mask_out[mw0 + tid] = m;
-    }
-    const uint32_t NC = 4 * R;
-    const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
-    const uint32_t cinc = wave_incl_scan(cc);
-    uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
-    if (ctile > CW_LDS / VB) ctile = CW_LDS / VB; // never exceeds the tile; bounds the write loop
-    uint32_t otile = 0;
-    if (OVF) otile = wave_sum32(lane < NC ? s_covf[lane] : 0);
-    // packed outputs: stage the selected values, compacted
-#pragma unroll
-    for (uint32_t r = 0; r < RM; r++) {
-        if (r >= R) continue;
-        const uint32_t ci = wave * R + r;
-        uint32_t pos = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r];
-        const uint32_t gg = g[r];
-        if (WIDE) {
-            if (gg & 1) s_vals[pos] = (V)(x[r] & vm);
-        }
-        else if (gg) {
-            const uint32_t lo = (uint32_t)x[r], hi = (uint32_t)(x[r] >> 32);
-            for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
-                if ((gg >> s) & 1) s_vals[pos++] = (V)(field32(lo, hi, sh) & vm32);
-        }
-    }
-    __syncthreads();
-    if (wave == 0) {
-        const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
-        tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
-    }
-    __syncthreads();
-    tile_write<OK>(a, s_vals, s_P, s_carry, ctile, t == a.T - 1, tid);
-}
-
 // ------------------------------------------------------------ direct kernel
 //
 // Container outputs (1/2/4/8-byte) from slab / container inputs. A block is
@@ -414,29 +246,12 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_words(CompressArgs a)
 // same LDS (now free) at the narrowest width that holds them; wave 0 does
 // its own share after the look-back. Once the tile's output offset P is
 // known, the block copies the compacted run to the output with vector stores.
+constexpr uint32_t CW_MASKW = 256; // output mask words per tile (4*R*eps)
 constexpr uint32_t CD_RMAX = 8;
 constexpr uint32_t CD_WAVES = 4;
 constexpr uint32_t CD_THREADS = CD_WAVES * 64;
 constexpr uint32_t CD_WORDS = CD_WAVES * 64 * CD_RMAX;  // input words per tile (max)
 constexpr uint32_t CD_STAGE = 20480;                    // LDS bytes: input words, then compacted values
-
-// Selection bits of a word's eps fields (field s -> bit s; field s is bits
-// [s*fw, s*fw + bw)). Shifting the word left by 64 - s*fw - bw puts field s
-// at the top of the high half with lower fields' bits beneath it; the range
-// test is scaled by 2^(32-bw) so those bits cannot change its outcome:
-//   v in [lo, lo + span]  <=>  (y - lo*2^k) <= (span + 1)*2^k - 1,  k = 32 - bw.
-// Compare and accumulate are one v_cmp + one v_addc (carry = the compare).
-__device__ inline uint32_t select_fields(uint64_t x, uint32_t eps, uint32_t fw, uint32_t bw, uint32_t LO, uint32_t RG)
-{
-    uint32_t gr = 0; // bits in reverse field order
-    for (uint32_t s = 0, sh = 64 - bw; s < eps; s++, sh -= fw) {
-        const uint32_t t = (uint32_t)((x << sh) >> 32) - LO;
-        uint32_t ng;
-        asm("v_cmp_ge_u32 vcc, %2, %1\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc" : "=v"(ng) : "v"(t), "s"(RG), "v"(gr) : "vcc");
-        gr = ng;
-    }
-    return __builtin_bitreverse32(gr) >> (32 - eps);
-}
 
 // Copy values s_vals[0, cnt) to out[P, P + cnt) (capacity-checked), four
 // per thread step with one vector store where the output is aligned.
@@ -477,16 +292,16 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
         if (P + i < cap) out[P + i] = (C)s_vals[i];
 }
 
-template <bool PRED, bool OVF, int VB, bool WIDE>
+template <int OK, bool PRED, bool OVF, int VB, bool WIDE>
 __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
 {
-    constexpr uint32_t F = OVF ? LB_OVF : 0;
+    constexpr uint32_t F = CsFlags<OK, OVF>::F;
     using V = typename VT<VB>::T;
     typedef __attribute__((address_space(3))) void* lds_ptr;
     __shared__ uint64_t s_buf[CD_STAGE / 8]; // input words, then compacted values
     __shared__ uint64_t s_mask[CW_MASKW];
     __shared__ uint32_t s_ccnt[4 * CD_RMAX], s_covf[4 * CD_RMAX];
-    __shared__ uint64_t s_P;
+    __shared__ uint64_t s_P, s_carry;
     __shared__ uint32_t s_epoch, s_abort;
     uint64_t* const s_x = s_buf;
     V* const s_vals = (V*)s_buf;
@@ -501,6 +316,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     const uint32_t NMW = CD_WAVES * R * eps;                 // mask words per tile (<= CW_MASKW)
     const uint64_t mw0 = (w0 * eps) >> 6;
     const bool last = t == a.T - 1;
+    const bool pow2 = (eps & (eps - 1)) == 0;
 
     // ---- loads: the tile's words -> LDS (wave w: local words [64Rw, 64R(w+1)))
     uint32_t ep = 0, ab = 0;
@@ -533,7 +349,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             if (tid < NMW && mw0 + tid < nmw) m = a.mask_in[mw0 + tid];
             if (tid < NMW) s_mask[tid] = m;
         }
-        else if (a.mask_out && tid < NMW)
+        else if (a.mask_out && !pow2 && tid < NMW)
             s_mask[tid] = 0;
         __builtin_amdgcn_s_waitcnt(0); // this wave's global->LDS copies
     }
@@ -565,7 +381,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             else if (WIDE)
                 gg = a.pr(x[r] & a.in.mask);
             else
-                gg = select_fields(x[r], eps, fw, bw, a.lo_top, a.rg_top) ^ a.inv_bits;
+                gg = select_fields(x[r], eps, fw, bw, a.tp);
             if (last) { // elements past n (only in the last tile)
                 const uint64_t e0 = (w0 + wl) * eps;
                 const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
@@ -584,10 +400,19 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             const uint32_t inc = wave_incl_scan(c);
             lp[r] = inc - c;
             if (lane == 63) s_ccnt[ci] = inc;
-            if (PRED && a.mask_out && gg) {
-                const uint32_t w = p >> 6, off = p & 63;
-                atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
-                if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
+            if (PRED && a.mask_out) {
+                if (pow2) { // the lanes of one mask word are an aligned segment: DPP OR
+                    const uint32_t S = 64 / eps;
+                    const uint64_t m = eps == 1 ? ballot(gg & 1)
+                                                : segment_or((uint64_t)gg << __umul24(lane & (S - 1), eps), S);
+                    const uint64_t mi = mw0 + (uint64_t)ci * eps + lane / S;
+                    if ((lane & (S - 1)) == S - 1 && mi < nmw) a.mask_out[mi] = m;
+                }
+                else if (gg) {
+                    const uint32_t w = p >> 6, off = p & 63;
+                    atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
+                    if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
+                }
             }
         }
     }
@@ -604,12 +429,13 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
     const uint32_t cinc = wave_incl_scan(cc);
     const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
-    if (a.mask_out && tid < NMW && mw0 + tid < nmw) {
+    if (a.mask_out && (!PRED || !pow2) && tid < NMW && mw0 + tid < nmw) {
         uint64_t m = s_mask[tid];
         if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
         a.mask_out[mw0 + tid] = m;
     }
-    if (lead) {
+    // container outputs: wave 0 looks back first (its words stay in registers)
+    if (OK == K_STRIDED && lead) {
         const Lookback<F> L = make_lookback<F>(a, s_epoch);
         L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
         const uint32_t otile = OVF ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
@@ -626,21 +452,36 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
         if (WIDE) *dst = (V)(x[r] & a.in.mask & a.out.mask);
         else
             for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
-                if (gg & (1u << s)) *dst++ = (V)((uint32_t)(x[r] >> sh) & vm32);
+                if (__builtin_amdgcn_ubfe(gg, s, 1)) *dst++ = (V)((uint32_t)(x[r] >> sh) & vm32);
     }
-    __syncthreads();
-    const uint64_t P = s_P;
-    switch (a.out.cbytes) {
-        case 1: copy_out<uint8_t>(a, s_vals, P, ctile, tid); break;
-        case 2: copy_out<uint16_t>(a, s_vals, P, ctile, tid); break;
-        case 4: copy_out<uint32_t>(a, s_vals, P, ctile, tid); break;
-        default: copy_out<uint64_t>(a, s_vals, P, ctile, tid);
+    if (OK == K_STRIDED) {
+        __syncthreads();
+        const uint64_t P = s_P;
+        switch (a.out.cbytes) {
+            case 1: copy_out<uint8_t>(a, s_vals, P, ctile, tid); break;
+            case 2: copy_out<uint16_t>(a, s_vals, P, ctile, tid); break;
+            case 4: copy_out<uint32_t>(a, s_vals, P, ctile, tid); break;
+            default: copy_out<uint64_t>(a, s_vals, P, ctile, tid);
+        }
+        if (last && tid == 0) {
+            uint64_t K = P + ctile;
+            if (K > a.cap) K = a.cap;
+            const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
+            for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
+        }
     }
-    if (last && tid == 0) {
-        uint64_t K = P + ctile;
-        if (K > a.cap) K = a.cap;
-        const uint64_t b0 = K * a.out.cbytes, b1 = (b0 + 7) & ~7ull;
-        for (uint64_t b = b0; b < b1; b++) ((uint8_t*)a.out.p)[b] = 0;
+    else {
+        // packed outputs: the tile's window needs its staged values
+        __syncthreads();
+        if (lead) {
+            const Lookback<F> L = make_lookback<F>(a, s_epoch);
+            L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
+            const uint32_t otile = OVF ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
+            const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
+            tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
+        }
+        __syncthreads();
+        tile_write<OK>(a, s_vals, s_P, s_carry, ctile, last, tid);
     }
 }
 
@@ -743,38 +584,20 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
 template <int OK, bool PRED, bool OVF>
 static void launch_words(const CompressArgs& a, uint32_t vb, bool wide, hipStream_t s)
 {
-    const dim3 grid(a.T), block(CS_THREADS);
-    if constexpr (OK == K_STRIDED) {
-        const dim3 dblock(CD_THREADS);
-        if (wide) {
-            switch (vb) {
-                case 1: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 1, true>), grid, dblock, 0, s, a); break;
-                case 2: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 2, true>), grid, dblock, 0, s, a); break;
-                case 4: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 4, true>), grid, dblock, 0, s, a); break;
-                default: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 8, true>), grid, dblock, 0, s, a);
-            }
-        }
-        else {
-            switch (vb) {
-                case 1: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 1, false>), grid, dblock, 0, s, a); break;
-                case 2: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 2, false>), grid, dblock, 0, s, a); break;
-                default: hipLaunchKernelGGL((k_compress_direct<PRED, OVF, 4, false>), grid, dblock, 0, s, a);
-            }
-        }
-    }
-    else if (wide) {
+    const dim3 grid(a.T), block(CD_THREADS);
+    if (wide) {
         switch (vb) {
-            case 1: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 1, true>), grid, block, 0, s, a); break;
-            case 2: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 2, true>), grid, block, 0, s, a); break;
-            case 4: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 4, true>), grid, block, 0, s, a); break;
-            default: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 8, true>), grid, block, 0, s, a);
+            case 1: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 1, true>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 2, true>), grid, block, 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 4, true>), grid, block, 0, s, a); break;
+            default: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 8, true>), grid, block, 0, s, a);
         }
     }
     else {
         switch (vb) {
-            case 1: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 1, false>), grid, block, 0, s, a); break;
-            case 2: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 2, false>), grid, block, 0, s, a); break;
-            default: hipLaunchKernelGGL((k_compress_words<OK, PRED, OVF, 4, false>), grid, block, 0, s, a);
+            case 1: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 1, false>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 2, false>), grid, block, 0, s, a); break;
+            default: hipLaunchKernelGGL((k_compress_direct<OK, PRED, OVF, 4, false>), grid, block, 0, s, a);
         }
     }
 }
@@ -858,7 +681,7 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     if (words) {
         a.fw = ci.kind == K_SLAB ? ci.bits : 8 * ci.cbytes;
         a.eps = 64 / a.fw;
-        if (co.kind == K_STRIDED) {
+        {
             // mask words per tile 4*R*eps <= 256; compacted values 256*R*eps*VB <= CD_STAGE
             const uint32_t vbh = value_bytes(ci.bits < co.bits ? ci.bits : co.bits);
             a.R = 64 / a.eps;
@@ -868,17 +691,8 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
             if (a.R < 1) a.R = 1;
             a.lds16 = ((uintptr_t)ci.p & 15) == 0;
             a.out_vec = ((uintptr_t)co.p & (4 * co.cbytes - 1)) == 0;
-            if (pred && !mask && ci.bits <= 32) {
-                // (v - lo) <= span over top-aligned fields: scale by 2^(32 - bits)
-                const uint32_t k = 32 - ci.bits;
-                const uint64_t sp1 = a.pr.span >= ci.mask ? (ci.mask + 1) : a.pr.span + 1;
-                a.lo_top = (uint32_t)(a.pr.lo << k);
-                a.rg_top = (uint32_t)((sp1 << k) - 1);
-                a.inv_bits = a.pr.inv ? (uint32_t)lowmask(a.eps) : 0;
-            }
+            if (pred && !mask && ci.bits <= 32) a.tp = make_top_pred(a.pr, ci.bits, a.eps);
         }
-        else // staged values 256*R*eps*VB <= 16 KB
-            a.R = a.eps <= 8 ? 4 : (a.eps <= 16 ? 2 : 1);
         a.in_words = (ci.n + a.eps - 1) / a.eps;
         const uint64_t TW = (uint64_t)256 * a.R; // 4 (data) waves x R chunks x 64 words
         T = a.in_words ? (a.in_words + TW - 1) / TW : 1;

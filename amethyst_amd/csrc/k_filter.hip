@@ -5,10 +5,11 @@
 // filter_slabs.cuh:4-57). There each thread decodes 32 consecutive elements
 // serially and writes one u32 mask word; the baseline variant needs a 32x32
 // shared-memory bit transpose.
-// Here lane l of a wave decodes element base + 64j + l, so one wave64
-// ballot (a single v_cmp into an SGPR pair) *is* mask word j -- no transpose,
-// no shared memory. Lane j keeps ballot j and the wave writes its 16 words
-// with one coalesced store.
+// Here slab and container inputs go word-per-lane (k_filter_words: all
+// fields of a word selected with 4 VALU each, mask words combined across
+// lanes by DPP or in LDS); bitstream inputs go element-per-lane (k_filter: lane l
+// of a wave decodes element base + 64j + l, so one wave64 ballot -- a single
+// v_cmp into an SGPR pair -- *is* mask word j; no transpose).
 //
 // Sum reference: kernel_hop_slabs / kernel_hop_nogaps (src/kernels/hop.cuh:7-76),
 // one global atomicAdd per thread. Here each lane decodes whole packed words
@@ -68,6 +69,86 @@ __global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_
 #pragma unroll
         for (uint32_t j = 0; j < FT_J; j++) f[j] = nf[j];
     }
+}
+
+
+// Word-per-lane filter for slab and container inputs: a block owns a tile
+// of 256*R input words; each lane issues its R word loads at once and forms
+// each word's selection bits with select_fields (4 VALU per field). Mask
+// words: one field per word -> the chunk's ballot; a power-of-two number of
+// fields per word -> the lanes feeding one mask word are an aligned segment
+// of 64/eps lanes, combined with a DPP OR-scan and stored by its last lane;
+// other widths -> OR-ed into the tile's mask words in LDS (at most 21 lanes
+// per word) and stored coalesced. One tile per block; no block waits on another.
+struct FilterWordsArgs {
+    Col in;
+    Pred pr;
+    TopPred tp;
+    uint32_t fw, eps, R, pow2;
+    uint64_t in_words;
+    uint64_t* mask_out;
+};
+
+constexpr uint32_t FW_RMAX = 8;
+constexpr uint32_t FW_MASKW = 1024; // LDS mask words per tile (non-power-of-two eps)
+
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
+{
+    __shared__ uint64_t s_mask[FW_MASKW];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t t = blockIdx.x;
+    const uint32_t fw = a.fw, eps = a.eps, R = a.R, bw = a.in.bits;
+    const bool lds = !WIDE && !a.pow2;
+    const uint64_t n = a.in.n, nw = a.in_words;
+    const uint64_t w0 = t * (uint64_t)(256 * R);
+    const uint64_t nmw = (n + 63) >> 6;
+    const uint32_t NMW = 4 * R * eps;
+    const uint64_t mw0 = (w0 * eps) >> 6;
+    uint64_t x[FW_RMAX];
+#pragma unroll
+    for (uint32_t r = 0; r < FW_RMAX; r++) {
+        x[r] = 0;
+        const uint64_t wi = w0 + (uint64_t)(wave * R + r) * 64 + lane;
+        if (r < R && wi < nw) x[r] = a.in.p[wi];
+    }
+    if (lds) {
+        for (uint32_t i = tid; i < NMW; i += 256) s_mask[i] = 0;
+        __syncthreads();
+    }
+    const bool last = t == gridDim.x - 1;
+    const uint32_t leps = __umul24(lane, eps);
+    const uint32_t S = 64 / eps; // lanes per mask word (power-of-two eps)
+#pragma unroll
+    for (uint32_t r = 0; r < FW_RMAX; r++) {
+        if (r >= R) continue;
+        const uint32_t ci = wave * R + r;
+        uint32_t gg = WIDE ? (uint32_t)a.pr(x[r] & a.in.mask) : select_fields(x[r], eps, fw, bw, a.tp);
+        if (last) {
+            const uint64_t e0 = (w0 + ci * 64 + lane) * eps;
+            const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
+            gg &= (uint32_t)lowmask(nv);
+        }
+        const uint64_t cmw = mw0 + (uint64_t)ci * eps; // first mask word of the chunk
+        if (eps == 1) { // the chunk's mask word is the ballot
+            const uint64_t m = ballot(gg & 1);
+            if (lane == 0 && cmw < nmw) a.mask_out[cmw] = m;
+        }
+        else if (!lds) {
+            const uint64_t m = segment_or((uint64_t)gg << __umul24(lane & (S - 1), eps), S);
+            const uint64_t mi = cmw + lane / S;
+            if ((lane & (S - 1)) == S - 1 && mi < nmw) a.mask_out[mi] = m;
+        }
+        else if (gg) {
+            const uint32_t p = ci * 64 * eps + leps, w = p >> 6, off = p & 63;
+            atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
+            if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
+        }
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (uint32_t i = tid; i < NMW; i += 256)
+        if (mw0 + i < nmw) a.mask_out[mw0 + i] = s_mask[i];
 }
 
 // ---- sum: lane-per-word decode, per-block partials, then one final block.
@@ -197,6 +278,30 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
     if (T >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
     const Pred pr = make_pred(pred, c.mask);
     hipStream_t s = (hipStream_t)stream;
+    if (c.kind != K_BITSTREAM && !(c.kind == K_SLAB && c.bits == 1)) {
+        FilterWordsArgs a{};
+        a.in = c;
+        a.pr = pr;
+        a.fw = c.kind == K_SLAB ? c.bits : 8 * c.cbytes;
+        a.eps = 64 / a.fw;
+        a.pow2 = (a.eps & (a.eps - 1)) == 0;
+        a.in_words = (c.n + a.eps - 1) / a.eps;
+        // enough blocks to keep every CU busy to the end (~8192), at most
+        // FW_RMAX words per lane, and LDS mask words 4*R*eps <= FW_MASKW
+        uint64_t r = a.in_words / (256ull * 8192);
+        if (r < 2) r = 2;
+        if (!a.pow2 && r > FW_MASKW / (4 * a.eps)) r = FW_MASKW / (4 * a.eps);
+        a.R = (uint32_t)(r > FW_RMAX ? FW_RMAX : r);
+        if (c.bits <= 32) a.tp = make_top_pred(pr, c.bits, a.eps);
+        a.mask_out = mask_out;
+        const uint64_t TW = 256ull * a.R;
+        const uint64_t TWB = (a.in_words + TW - 1) / TW;
+        if (TWB >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
+        if (c.bits > 32) hipLaunchKernelGGL(k_filter_words<true>, dim3((uint32_t)TWB), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_filter_words<false>, dim3((uint32_t)TWB), dim3(256), 0, s, a);
+        return check_launch();
+    }
+    if (c.kind == K_SLAB) c.kind = K_BITSTREAM; // 1-bit slabs are dense streams
 #define FT_GO(IK, W)                                                                                        \
     {                                                                                                       \
         static uint32_t capb = 0;                                                                           \
