@@ -7,9 +7,10 @@
 // compile-time function pointer masked to the out-width (src/config.cuh:5-42).
 //
 // Here one lane also owns one output word (so every store is a coalesced,
-// whole-word write and dead/padding bits are written as zero), but input
-// elements are decoded straight from global memory through a per-block
-// TileReader: neighbouring lanes touch neighbouring input words, so each load
+// whole-word write and dead/padding bits are written as zero). Slab and
+// container columns go word-per-lane (k_binary_words: fields at wave-uniform
+// shifts); bitstream columns decode each element straight from global memory
+// through a per-block TileReader: neighbouring lanes touch neighbouring input words, so each load
 // instruction covers one or two 128-byte lines and the L1 absorbs the reuse
 // between lanes. Any input format x any output format; the operator is a
 // template argument so the switch folds away.
@@ -94,9 +95,92 @@ __global__ __launch_bounds__(256) void k_elementwise(Col a, Col b, Col out)
     }
 }
 
+// Word-per-lane binary op for slab and container columns (a container
+// column is a slab of fw = 8*cbytes-bit fields). Lane q builds output word q
+// from its eps_o elements. ALIGNED: both inputs hold the same elements in
+// word q (eps_i == eps_o), so the lane reads one word of each and applies the
+// operator field by field at wave-uniform shifts. Otherwise the lane's first
+// input word and field come from a per-block base and a 32-bit reciprocal
+// multiply, and it reads two words of each input (eps_o <= eps_i: an output
+// word's elements span at most two input words).
+struct BinWords {
+    Col a, b, out;
+    uint32_t fi, eps_i, fo, eps_o;
+    uint64_t words_i, words_o;
+};
+
+template <int OP, bool ALIGNED>
+__global__ __launch_bounds__(256) void k_binary_words(BinWords g)
+{
+    const uint64_t u0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t q = u0 + threadIdx.x;
+    if (q >= g.words_o) return;
+    const uint64_t n = g.out.n;
+    const uint32_t fi = g.fi, fo = g.fo, ei = g.eps_i, eo = g.eps_o;
+    const uint64_t im = g.a.mask, om = g.out.mask;
+    const uint64_t e0 = q * eo;
+    const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
+    uint64_t word = 0;
+    if (ALIGNED) {
+        const uint64_t x = g.a.p[q], y = g.b.p[q];
+        for (uint32_t j = 0, si = 0, so = 0; j < eo; j++, si += fi, so += fo) {
+            const uint64_t r = binop<OP>((x >> si) & im, (y >> si) & im) & om;
+            word |= (j < cnt ? r : 0) << so;
+        }
+    }
+    else {
+        // block base: element u0*eo lies in input word qb at field rb
+        const uint64_t t0 = u0 * eo;
+        const uint64_t qb = t0 / ei;
+        const uint32_t rb = (uint32_t)(t0 - qb * ei);
+        const uint32_t rel = threadIdx.x * eo + rb; // < 2^24
+        const uint32_t dq = ei == 1 ? rel : __umulhi(rel, g.a.eps_magic);
+        const uint32_t s0 = rel - dq * ei;
+        const uint64_t qi = qb + dq;
+        const uint64_t x0 = g.a.p[qi], y0 = g.b.p[qi];
+        const bool two = s0 + eo > ei && qi + 1 < g.words_i;
+        const uint64_t x1 = two ? g.a.p[qi + 1] : 0, y1 = two ? g.b.p[qi + 1] : 0;
+        for (uint32_t j = 0, so = 0; j < eo; j++, so += fo) {
+            const uint32_t s = s0 + j;
+            const bool hi = s >= ei;
+            const uint32_t sh = (hi ? s - ei : s) * fi;
+            const uint64_t va = ((hi ? x1 : x0) >> sh) & im, vb = ((hi ? y1 : y0) >> sh) & im;
+            const uint64_t r = binop<OP>(va, vb) & om;
+            word |= (j < cnt ? r : 0) << so;
+        }
+    }
+    g.out.p[q] = word;
+}
+
+template <int OP>
+static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s)
+{
+    auto field = [](const Col& c) { return c.kind == K_SLAB ? c.bits : 8 * c.cbytes; };
+    if (a.kind == K_BITSTREAM || o.kind == K_BITSTREAM) return false;
+    BinWords g{};
+    g.a = a;
+    g.b = b;
+    g.out = o;
+    g.fi = field(a);
+    g.eps_i = 64 / g.fi;
+    g.fo = field(o);
+    g.eps_o = 64 / g.fo;
+    if (g.eps_o > g.eps_i) return false; // narrower output: the general kernel
+    g.a.eps = g.eps_i;
+    g.a.eps_magic = g.eps_i > 1 ? (uint32_t)((1ull << 32) / g.eps_i + 1) : 0;
+    g.words_i = (a.n + g.eps_i - 1) / g.eps_i;
+    g.words_o = (o.n + g.eps_o - 1) / g.eps_o;
+    const dim3 grid((uint32_t)((g.words_o + 255) / 256));
+    if (g.eps_o == g.eps_i) hipLaunchKernelGGL((k_binary_words<OP, true>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_binary_words<OP, false>), grid, dim3(256), 0, s, g);
+    return true;
+}
+
 template <int OP>
 static void launch_ew(const Col& a, const Col& b, const Col& o, hipStream_t s)
 {
+    if constexpr (OP != OP_COPY)
+        if (launch_words<OP>(a, b, o, s)) return;
     uint64_t units = o.kind == K_STRIDED ? o.n : (o.kind == K_SLAB ? (o.n + o.eps - 1) / o.eps : (o.n * o.bits + 63) / 64);
     dim3 grid((uint32_t)((units + EW_UNITS - 1) / EW_UNITS));
 #define EW_CASE(AK, OK) hipLaunchKernelGGL((k_elementwise<AK, OK, OP>), grid, dim3(256), 0, s, a, b, o)

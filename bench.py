@@ -107,7 +107,16 @@ def cpu_baseline(am, torch, col, pred_c):
         res[threads] = (b / t / 1e9, t)
     st = res[1]
     mt_threads = max(res)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round(st[0], 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(), "threads_parallel": mt_threads,
             "sample": f"first 2^24 of the 2^28 w=11 slab column (filter v<1024 + compressstore to u32 + mask), "
                       f"single-thread faithful per-element loop, median of 3 after 1 warm-up ({st[1]*1e3:.1f} ms)",
             "parallel": {"value": round(res[mt_threads][0], 3), "cores": mt_threads,
@@ -403,6 +412,12 @@ def main():
         ops += matmul_leg(am, torch, timer, args.reps)
         print("batched", file=sys.stderr, flush=True)
         extra["batched"] = batched_leg(am, torch, args.reps)
+    # measured copy peak on this box: a plain 1 GiB device-to-device copy
+    src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    copy_ms = timer.median_ms(lambda: dst.copy_(src), max(5, args.reps))
+    copy_gbs = gbytes(2 * src.numel() * 8, copy_ms)
+    del src, dst
     print("cpu baseline", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(am, torch, col, c)
 
@@ -425,6 +440,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_compress_direct<PRED=1,OVF=0,VB=2,WIDE=0>", "avg_kernel_ms": round(avg_kernel, 5)},
+        "copy_peak": {"value": round(copy_gbs, 1), "unit": "GB/s",
+                      "kernel": "torch copy_ of 1 GiB int64 (device to device)",
+                      "headline_frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "ops": ops,
     }

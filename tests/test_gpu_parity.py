@@ -189,6 +189,32 @@ def test_compress_many_tiles(am, oracle_mod):
     ws.check()
 
 
+def test_config_c1_exact(am, oracle_mod):
+    """Config C1 as specified: N = 2^24 values rand() & 0x7FF from the PCG
+    stream of seed 42, predicate v < 1024, output u32 + bitmask, slab and
+    bitstream inputs -- generation, count, mask and output byte-exact with the
+    oracle. Also the reference's own predicate v >= 2^(w-1) (the complement)."""
+    n, w = 1 << 24, 11
+    vals = oracle_mod.prng(42, n).astype(np.uint64) & np.uint64(0x7FF)
+    ws = am.Workspace(n)
+    for fmt in (1, 2):
+        col = am.gen_uniform(42, n, w, fmt)
+        packed = oracle_mod.pack(fmt, w, vals)
+        assert np.array_equal(col.host_words(), packed), fmt
+        for op, c in ((0, 1024), (3, 1 << (w - 1))):
+            say("config-c1", fmt, op, c)
+            mask = oracle_mod.filter(fmt, w, packed, n, op, c)
+            out, stats, mo = am.compressstore(col, 32, 4, pred=(op, c), want_mask=True, ws=ws)
+            k, exp_words, ovf = oracle_mod.compressstore(fmt, w, packed, n, mask, 4, 32)
+            st = am.u64(stats)
+            assert int(st[0]) == k and int(st[1]) == 0
+            assert abs(k - n // 2) < 20000  # selectivity ~0.5
+            assert np.array_equal(am.u64(mo)[:(n + 63) // 64], mask)
+            used = oracle_mod.packed_words(4, 32, k)
+            assert np.array_equal(out.host_words()[:used], exp_words[:used])
+    ws.check()
+
+
 def test_compress_capacity_truncation(am, oracle_mod):
     rng = np.random.default_rng(5)
     ws = am.Workspace(1 << 16)
