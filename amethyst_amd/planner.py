@@ -265,10 +265,12 @@ def train(X, T, epochs=400, lr=3e-3, seed=0):
 class Planner:
     """Loads trained weights and picks plans with the HIP MLP kernel."""
 
-    def __init__(self, path=WEIGHTS, min_margin=0.02):
+    def __init__(self, path=WEIGHTS, min_margin=None):
         import amethyst_amd as am
         torch = am._torch()
         z = np.load(path)
+        if min_margin is None:
+            min_margin = float(z["margin"]) if "margin" in z.files else 0.02
         self.P = int(z["nplans"])
         self.w = torch.tensor(z["weights"], device="cuda")
         self.mean = torch.tensor(z["mean"], device="cuda")
@@ -335,14 +337,42 @@ def _collect(args):
     print("wrote", args.out)
 
 
+def choose_with_margin(scores, margin):
+    """Host restatement of k_plan_mlp's rule: best plan when it beats the
+    runner-up by at least `margin`, else the analytic plan."""
+    s = np.sort(scores, axis=1)
+    arg = scores.argmin(axis=1)
+    return np.where(s[:, 1] - s[:, 0] >= margin, arg, ANALYTIC)
+
+
+def regret(T, choice):
+    return float(T[np.arange(len(T)), choice].sum() / T.min(axis=1).sum() - 1)
+
+
+MARGINS = (0.0, 0.005, 0.01, 0.02, 0.05, 0.1)
+
+
 def _train(args):
-    z = np.load(args.data)
-    w = train(z["X"], z["T"], epochs=args.epochs)
+    X, T = [], []
+    for path in args.data.split(","):
+        z = np.load(path)
+        X.append(z["X"])
+        T.append(z["T"])
+    X, T = np.concatenate(X), np.concatenate(T)
+    # hold out 20 % (every 5th column) to choose the confidence margin
+    val = np.arange(len(T)) % 5 == 0
+    w = train(X[~val], T[~val], epochs=args.epochs)
+    sv = mlp_reference(w, X[val])
+    res = {m: regret(T[val], choose_with_margin(sv, m)) for m in MARGINS}
+    margin = min(res, key=res.get)
+    print("validation regret by margin: " + ", ".join(f"{m}: {100 * r:.2f}%" for m, r in res.items()) +
+          f"; analytic {100 * regret(T[val], np.full(val.sum(), ANALYTIC)):.2f}%")
+    # final fit on all columns, with the chosen margin stored beside the weights
+    w = train(X, T, epochs=args.epochs)
+    w["margin"] = np.float32(margin)
     np.savez(args.out, **w)
-    T = z["T"]
-    pred = mlp_reference(w, z["X"]).argmin(axis=1)
-    regret = T[np.arange(len(T)), pred].sum() / T.min(axis=1).sum() - 1
-    print(f"trained on {len(T)} columns; training-set regret {100 * regret:.2f}%")
+    tr = regret(T, choose_with_margin(mlp_reference(w, X), margin))
+    print(f"trained on {len(T)} columns; margin {margin}; training-set regret {100 * tr:.2f}%")
 
 
 def _evaluate(args):
@@ -362,7 +392,8 @@ def _evaluate(args):
         ref = mlp_reference(zw, fr)
         times = time_plans(a, b, pred, fr[0], reps=args.reps)
         rows.append({"spec": list(map(int, spec)), "times": times.tolist(), "choice": int(choice[0]),
-                     "mlp_match": bool(np.allclose(sc[0], ref[0], rtol=1e-3, atol=1e-3))})
+                     "mlp_match": bool(np.allclose(sc[0], ref[0], rtol=1e-3, atol=1e-3)),
+                     "feat": fr[0].tolist(), "scores": sc[0].tolist()})
         del a, b
         if k % 200 == 0:
             print(f"evaluate {k}/{args.n} {time.time() - t0:.0f}s", flush=True)
@@ -370,8 +401,14 @@ def _evaluate(args):
     T = np.array([r["times"] for r in rows])
     ch = np.array([r["choice"] for r in rows])
     best = T.min(axis=1)
+    S = np.array([r["scores"] for r in rows])
+    np.savez(os.path.splitext(args.out)[0] + ".npz", X=np.array([r["feat"] for r in rows], np.float32), T=T,
+             S=S, choice=ch, specs=np.array([r["spec"] for r in rows], np.int64))
     res = {
         "columns": len(rows),
+        "margin": pl.min_margin,
+        "regret_by_margin": {str(m): regret(T, choose_with_margin(S, m)) for m in MARGINS},
+        "regret_best_single_plan": float(min(T[:, p].sum() for p in range(NPLANS)) / best.sum() - 1),
         "regret_learned": float(T[np.arange(len(T)), ch].sum() / best.sum() - 1),
         "regret_analytic": float(T[:, ANALYTIC].sum() / best.sum() - 1),
         "mean_col_regret_learned": float(np.mean(T[np.arange(len(T)), ch] / best - 1)),
@@ -396,7 +433,7 @@ def main(argv=None):
     c.add_argument("--reps", type=int, default=3)
     c.add_argument("--out", default="gpurun_out/plan_train.npz")
     t = sub.add_parser("train")
-    t.add_argument("--data", required=True)
+    t.add_argument("--data", required=True, help="one or more .npz files, comma-separated")
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--out", default=WEIGHTS)
     e = sub.add_parser("evaluate")
@@ -404,7 +441,7 @@ def main(argv=None):
     e.add_argument("--seed", type=int, default=900000)
     e.add_argument("--nmax", type=int, default=1 << 26)
     e.add_argument("--reps", type=int, default=2)
-    e.add_argument("--margin", type=float, default=0.02)
+    e.add_argument("--margin", type=float, default=None)
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--out", default="gpurun_out/plan_eval.json")
     args = ap.parse_args(argv)
