@@ -259,15 +259,32 @@ def test_binary_ops(am, oracle_mod):
 def test_sum(am, oracle_mod):
     rng = np.random.default_rng(8)
     ws = am.Workspace(0)
-    for bits in (1, 4, 11, 20, 32, 33, 64):
+    for bits in (1, 2, 3, 4, 5, 8, 11, 16, 20, 21, 31, 32, 33, 64):
         for n in (0, 1, 4095, 4097, 1000003):
             vals = rand_vals(rng, n, bits)
-            for fmt in (0, 1, 2, 3):
+            for fmt in (0, 1, 2, 3, 4):
+                if not fmt_ok(fmt, bits):
+                    continue
                 say("sum", bits, n, fmt)
                 col = upload(am, oracle_mod, vals, bits, fmt)
                 got = int(am.u64(am.sum(col, ws=ws))[0])
                 exp = oracle_mod.sum(fmt, bits, oracle_mod.pack(fmt, bits, vals), n)
                 assert got == exp, (bits, n, fmt)
+
+
+def test_sum_swar_saturated(am, oracle_mod):
+    """Every field at its maximum, on columns long enough that each lane
+    accumulates its slots up to the flush bound (K = 2^w words for slabs):
+    the sum must be exactly n * (2^w - 1)."""
+    ws = am.Workspace(0)
+    for bits, n in ((2, 1 << 26), (3, 21 * (1 << 22)), (4, 1 << 27)):
+        eps = 64 // bits
+        words = (n + eps - 1) // eps
+        word = np.uint64((1 << (eps * bits)) - 1) if eps * bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+        say("sum-saturated", bits, n)
+        col = am.Column.from_host_words(np.full(words, word, dtype=np.uint64), n, bits, 1)
+        got = int(am.u64(am.sum(col, ws=ws))[0])
+        assert got == n * ((1 << bits) - 1), (bits, n, got)
 
 
 def _width(v):
