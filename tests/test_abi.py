@@ -78,6 +78,10 @@ def test_status_strings_and_errors(L):
     import amethyst_amd as am
     assert L.am_status_string(0) == b"ok"
     assert L.am_status_string(2) == b"unsupported configuration"
+    for code in range(7):  # every am_status has its own message
+        assert L.am_status_string(code) != b"unknown status", code
+    assert L.am_status_string(7) == b"unknown status"
+    assert L.am_workspace_status(None) == 1  # null workspace: invalid, checked before any HIP call
     # invalid arguments are rejected on the host before any launch
     bad = am.am_column(None, 10, 11, am.SLAB)
     ok = am.am_column(None, 0, 11, am.SLAB)
