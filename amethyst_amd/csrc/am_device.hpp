@@ -329,12 +329,16 @@ __host__ __device__ inline uint64_t* ws_gbpart(void* ws) { return (uint64_t*)((u
 //   [63:48] epoch&0xffff [47:46] level [45:0] payload
 // (window low 46 bits; window high 17 bits | overflow << 17; sum low 46
 // bits; sum high 18 bits). Per tile: status, aggregate words, inclusive
-// words. Per group of 64 tiles: an aggregate of the whole group, published
-// by the group's last tile when its own walk over its 63 group-mates found
-// no inclusive value (otherwise that tile's inclusive value arrives just as
-// soon). A look-back therefore needs two dependent round trips: the
-// group-mates below the tile, then whole groups, 64 per step, stopping at
-// the nearest tile or group end whose inclusive value is known.
+// words; per group of 64 tiles the same, for the whole group. The group's
+// last tile publishes the group aggregate (as soon as its 63 group-mates'
+// aggregates are in, or during its own walk) and, with its own inclusive
+// value, the group's. A look-back reads, in one round trip, the tiles from
+// its predecessor down to the first tile of the previous group (so it
+// never waits for that group's words) together with the 64 groups older
+// than that (64 consecutive words), and stops at the nearest tile or group
+// whose inclusive value is known. Descriptor reads are device-coherent
+// loads that each move a whole memory line: the layout keeps a walk's lines
+// few (about a dozen).
 //
 // Every wait is bounded: after LB_WAIT_TICKS of the 100 MHz wall clock the
 // waiting wave sets the workspace's abort word and all waiters stop. An
@@ -382,29 +386,36 @@ struct Agg {
     uint64_t c, w, o, s;
 };
 
-// x (older) then y (newer)
+// x (older) then y (newer); only the fields a scan with flags F carries
+// are combined (the others stay zero)
+template <uint32_t F>
 __device__ inline Agg agg_combine(const Agg& x, const Agg& y, uint32_t bits)
 {
-    Agg r;
-    r.c = x.c + y.c;
-    r.w = win_combine(x.w, y.w, y.c, bits);
-    r.o = ovf_add(x.o, y.o);
-    r.s = x.s + y.s;
+    Agg r{x.c + y.c, 0, 0, 0};
+    if (F & LB_WIN) r.w = win_combine(x.w, y.w, y.c, bits);
+    if (F & LB_OVF) r.o = ovf_add(x.o, y.o);
+    if (F & LB_SUM) r.s = x.s + y.s;
     return r;
 }
 
+template <uint32_t F>
 __device__ inline Agg shfl_agg(const Agg& a, int m)
 {
-    return Agg{shfl_xor_u64(a.c, m), shfl_xor_u64(a.w, m), shfl_xor_u64(a.o, m), shfl_xor_u64(a.s, m)};
+    Agg r{shfl_xor_u64(a.c, m), 0, 0, 0};
+    if (F & LB_WIN) r.w = shfl_xor_u64(a.w, m);
+    if (F & LB_OVF) r.o = shfl_xor_u64(a.o, m);
+    if (F & LB_SUM) r.s = shfl_xor_u64(a.s, m);
+    return r;
 }
 
 // Ordered wave reduction: lane j holds element j, higher lanes are OLDER;
 // every lane returns the combination oldest..newest.
+template <uint32_t F>
 __device__ inline Agg wave_combine_ordered(Agg v, uint32_t lane, uint32_t bits)
 {
     for (int m = 1; m < 64; m <<= 1) {
-        const Agg o = shfl_agg(v, m);
-        v = (lane & m) ? agg_combine(v, o, bits) : agg_combine(o, v, bits);
+        const Agg o = shfl_agg<F>(v, m);
+        v = (lane & m) ? agg_combine<F>(v, o, bits) : agg_combine<F>(o, v, bits);
     }
     return v;
 }
@@ -414,7 +425,7 @@ struct Lookback {
     static constexpr uint32_t NX = ((F & (LB_WIN | LB_OVF)) ? 2 : 0) + ((F & LB_SUM) ? 2 : 0); // payload words
     static constexpr uint32_t NXA = NX ? NX : 1;                                           // array size
     static constexpr uint32_t TS = 1 + 2 * NX; // tile stride: status, aggregate words, inclusive words
-    static constexpr uint32_t GS = 1 + NX;     // group stride: status, aggregate words
+    static constexpr uint32_t GS = 1 + 2 * NX; // group stride: status, aggregate words, inclusive words
     uint64_t* tiles;  // [cap][TS]
     uint64_t* groups; // [cap/64 + 2][GS]
     uint64_t cap;     // tiles
@@ -495,107 +506,203 @@ struct Lookback {
 
     // Wave-level: publish tile t's aggregate, look back, publish its
     // inclusive value. Returns the exclusive prefix (all lanes).
+    //
+    // One round trip in the common case: the window of tiles from t-1 down
+    // to the first tile of the previous group (at most 127, two per lane)
+    // and the groups older than the window (64, one per lane) are read
+    // together; the walk stops at the nearest inclusive value in the window,
+    // else combines the window's aggregates with whole groups.
     __device__ Agg run(uint64_t t, const Agg& own, uint32_t lane) const
     {
-        const Agg zero{0, 0, 0, 0};
-        uint64_t* D = tile(t);
+        publish_agg(t, own, lane);
+        return walk(t, own, lane);
+    }
+    // tile t's aggregate (tile 0: its inclusive value)
+    __device__ void publish_agg(uint64_t t, const Agg& own, uint32_t lane) const
+    {
         if (lane == 0) {
+            uint64_t* D = tile(t);
             const uint32_t lvl = t == 0 ? 2 : 1;
             if (NX) put_x(D + 1 + (lvl - 1) * NX, lvl, own);
             st_rel(&D[0], lb_status(lvl, epoch, own.c));
         }
+    }
+    // The last tile of a group (t = 64g+63), its aggregate published: once
+    // its 63 group-mates have published theirs, publish the group's
+    // aggregate, so later look-backs need not wait for this tile's own
+    // look-back. Skipped when a mate already holds an inclusive value (this
+    // tile's inclusive value then follows within one round trip).
+    __device__ void publish_group_early(uint64_t t, const Agg& own, uint32_t lane) const
+    {
+        const Agg zero{0, 0, 0, 0};
+        const bool act = lane < 63;
+        const uint64_t* q = tile(act ? t - 1 - lane : t);
+        uint64_t st = 0, xa[NXA];
+        if (act) {
+            st = ld_acq(q);
+            load_x(q + 1, xa);
+        }
+        uint32_t it = 0;
+        uint64_t t0 = 0;
+        Agg v = zero;
+        for (;;) {
+            const bool ok = act && lb_status_ok(st, epoch);
+            if (ballot(ok && (st >> 62) == 2)) return;
+            const bool dec = !act || (ok && decode(xa, 1, st & CNT_MASK, v));
+            if (!ballot(!dec)) break;
+            if (!keep_waiting(it, t0)) return;
+            if (!dec) {
+                if (!ok) st = ld_acq(q);
+                load_x(q + 1, xa);
+            }
+        }
+        const Agg ga = agg_combine<F>(wave_combine_ordered<F>(act ? v : zero, lane, bits), own, bits);
+        if (lane == 0) {
+            if (NX) put_x(group(t >> 6) + 1, 1, ga);
+            st_rel(group(t >> 6), lb_status(1, epoch, ga.c));
+        }
+    }
+    // the look-back of a tile whose aggregate is published; publishes its
+    // inclusive value and returns the exclusive prefix (all lanes)
+    __device__ Agg walk(uint64_t t, const Agg& own, uint32_t lane) const
+    {
+        const Agg zero{0, 0, 0, 0};
+        uint64_t* D = tile(t);
         if (t == 0) return zero;
         const uint64_t g = t >> 6;
         const uint32_t below = (uint32_t)(t & 63);
-        Agg excl = zero;
-        bool done = false;
+        const uint64_t lo = g ? (g - 1) * 64 : 0;
+        const uint32_t nw = (uint32_t)(t - lo); // window tiles [lo, t): 1..127
+        // window index i: near i = lane (tile t-1-lane), far i = 64 + lane (tile t-65-lane)
+        const bool an = lane < nw, af = 64 + lane < nw;
+        const uint64_t* qn = tile(an ? t - 1 - lane : t);
+        const uint64_t* qf = tile(af ? t - 65 - lane : t);
+        // groups older than the window: lane j -> group g-2-j
+        const int64_t h0 = (int64_t)g - 2 - (int64_t)lane;
+        const uint64_t* gq = group(h0 >= 0 ? (uint64_t)h0 : 0);
+        // aggregate payloads come with the statuses; the one inclusive
+        // payload a walk uses (at its stop) is fetched once the stop is known
+        uint64_t sn = 0, sf = 0, gs = 0;
+        uint64_t xan[NXA], xaf[NXA], xg[NXA];
+        if (an) {
+            sn = ld_acq(qn);
+            load_x(qn + 1, xan);
+        }
+        if (af) {
+            sf = ld_acq(qf);
+            load_x(qf + 1, xaf);
+        }
+        if (h0 >= 0) {
+            gs = ld_acq(gq);
+            load_x(gq + 1, xg);
+        }
         bool gave_up = false;
-        if (below) {
-            // stage 1: lane j -> tile t-1-j (higher lanes older)
-            const bool act = lane < below;
-            const uint64_t* q = tile(act ? t - 1 - lane : t);
-            uint64_t st = 0, xa[NXA], xp[NXA];
-            if (act) {
-                st = ld_acq(q);
-                load_x(q + 1, xa);
-                load_x(q + 1 + NX, xp);
-            }
-            uint32_t it = 0, stop = below;
-            uint64_t t0 = 0, p2 = 0;
-            Agg v = zero;
-            for (;;) {
-                const bool sok = act && lb_status_ok(st, epoch);
-                p2 = ballot(sok && (st >> 62) == 2);
-                stop = p2 ? (uint32_t)__builtin_ctzll(p2) : below;
-                const bool need = lane < stop || (p2 && lane == stop);
-                bool ok = true;
-                if (need) ok = sok && decode(lane < stop ? xa : xp, lane < stop ? 1 : 2, st & CNT_MASK, v);
-                if (!ballot(need && !ok)) break;
-                if (!keep_waiting(it, t0)) {
-                    gave_up = true;
-                    break;
-                }
-                if (need && !ok) {
-                    if (!sok) st = ld_acq(q);
-                    load_x(q + 1, xa);
-                    load_x(q + 1 + NX, xp);
+        uint32_t it = 0, stop = nw;
+        uint64_t t0 = 0;
+        Agg vn = zero, vf = zero;
+        bool needn = false, needf = false;
+        for (;;) {
+            const bool okn = an && lb_status_ok(sn, epoch), okf = af && lb_status_ok(sf, epoch);
+            const uint64_t pn = ballot(okn && (sn >> 62) == 2), pf = ballot(okf && (sf >> 62) == 2);
+            stop = pn ? (uint32_t)__builtin_ctzll(pn) : (pf ? 64 + (uint32_t)__builtin_ctzll(pf) : nw);
+            const uint32_t fi = 64 + lane;
+            needn = an && (lane < stop || (lane == stop && stop < nw));
+            needf = af && (fi < stop || (fi == stop && stop < nw));
+            bool dn = true, df = true;
+            if (needn) {
+                if (lane < stop) dn = okn && decode(xan, 1, sn & CNT_MASK, vn);
+                else {
+                    uint64_t xp[NXA];
+                    load_x(qn + 1 + NX, xp);
+                    dn = decode(xp, 2, sn & CNT_MASK, vn);
                 }
             }
-            if (!(lane < stop || (p2 && lane == stop))) v = zero;
-            excl = wave_combine_ordered(v, lane, bits);
-            done = p2 != 0 || gave_up;
-            if (below == 63 && !done && lane == 0) {
-                // group aggregate of group g: tiles 64g .. 64g+63
-                const Agg ga = agg_combine(excl, own, bits);
+            if (needf) {
+                if (fi < stop) df = okf && decode(xaf, 1, sf & CNT_MASK, vf);
+                else {
+                    uint64_t xp[NXA];
+                    load_x(qf + 1 + NX, xp);
+                    df = decode(xp, 2, sf & CNT_MASK, vf);
+                }
+            }
+            if (!ballot((needn && !dn) || (needf && !df))) break;
+            if (!keep_waiting(it, t0)) {
+                gave_up = true;
+                break;
+            }
+            if (needn && !dn && lane < stop) {
+                if (!okn) sn = ld_acq(qn);
+                load_x(qn + 1, xan);
+            }
+            if (needf && !df && fi < stop) {
+                if (!okf) sf = ld_acq(qf);
+                load_x(qf + 1, xaf);
+            }
+        }
+        if (!needn) vn = zero;
+        if (!needf) vf = zero;
+        Agg excl = wave_combine_ordered<F>(vn, lane, bits);
+        if (nw > 64 && stop >= 64) excl = agg_combine<F>(wave_combine_ordered<F>(vf, lane, bits), excl, bits);
+        bool done = stop < nw || gave_up;
+        if (below == 63 && stop >= 63 && !gave_up) {
+            // group aggregate of group g: its other 63 tiles are near lanes 0..62
+            const Agg ga = agg_combine<F>(wave_combine_ordered<F>(lane < 63 ? vn : zero, lane, bits), own, bits);
+            if (lane == 0) {
                 if (NX) put_x(group(g) + 1, 1, ga);
                 st_rel(group(g), lb_status(1, epoch, ga.c));
             }
         }
-        // stage 2: whole groups, 64 per step (lane j -> group gb-1-j)
-        for (uint64_t gb = g; !done;) {
+        // whole groups, 64 per step (lane j -> group gb-1-j); the first step's words are in hand
+        for (uint64_t gb = g - 1; !done;) {
             const int64_t h = (int64_t)gb - 1 - (int64_t)lane;
             const bool act = h >= 0;
-            const uint64_t* lt = tile(act ? (uint64_t)h * 64 + 63 : 0);
-            const uint64_t* gg = group(act ? (uint64_t)h : 0);
-            uint64_t ls = 0, gs = 0, xl[NXA], xg[NXA];
-            if (act) {
-                ls = ld_acq(lt);
-                load_x(lt + 1 + NX, xl);
-                gs = ld_acq(gg);
-                load_x(gg + 1, xg);
+            if (gb != g - 1) {
+                gq = group(act ? (uint64_t)h : 0);
+                if (act) {
+                    gs = ld_acq(gq);
+                    load_x(gq + 1, xg);
+                }
             }
-            uint32_t it = 0, stop = 64;
-            uint64_t t0 = 0, p2 = 0;
+            uint32_t it2 = 0, gstop = 64;
+            uint64_t t2 = 0, p2 = 0;
             Agg vp = zero, vg = zero;
             for (;;) {
-                const bool hasP = act && lb_status_ok(ls, epoch) && (ls >> 62) == 2 &&
-                                  decode(xl, 2, ls & CNT_MASK, vp);
-                const bool hasG = act && lb_status_ok(gs, epoch) && decode(xg, 1, gs & CNT_MASK, vg);
-                p2 = ballot(!act || hasP);
-                stop = p2 ? (uint32_t)__builtin_ctzll(p2) : 64;
-                const bool need = lane < stop && !hasG;
+                // stop at the nearest group with its inclusive value
+                const bool gok = act && lb_status_ok(gs, epoch);
+                p2 = ballot(!act || (gok && (gs >> 62) == 2));
+                gstop = p2 ? (uint32_t)__builtin_ctzll(p2) : 64;
+                bool hasP = false;
+                if (lane == gstop && act) {
+                    uint64_t xl[NXA];
+                    load_x(gq + 1 + NX, xl);
+                    hasP = decode(xl, 2, gs & CNT_MASK, vp);
+                }
+                const bool hasG = gok && (gs >> 62) == 1 && decode(xg, 1, gs & CNT_MASK, vg);
+                const bool need = (lane < gstop && !hasG) || (lane == gstop && act && !hasP);
                 if (!ballot(need)) break;
-                if (!keep_waiting(it, t0)) {
+                if (!keep_waiting(it2, t2)) {
                     gave_up = true;
                     break;
                 }
-                if (need) {
-                    ls = ld_acq(lt);
-                    load_x(lt + 1 + NX, xl);
-                    gs = ld_acq(gg);
-                    load_x(gg + 1, xg);
+                if (need && lane < gstop) {
+                    gs = ld_acq(gq);
+                    load_x(gq + 1, xg);
                 }
             }
-            const Agg v = lane < stop ? vg : ((lane == stop && act) ? vp : zero);
-            const Agg part = wave_combine_ordered(v, lane, bits);
-            excl = agg_combine(part, excl, bits);
+            const Agg v = lane < gstop ? vg : ((lane == gstop && act) ? vp : zero);
+            const Agg part = wave_combine_ordered<F>(v, lane, bits);
+            excl = agg_combine<F>(part, excl, bits);
             done = p2 != 0 || gave_up;
             gb = gb >= 64 ? gb - 64 : 0;
         }
         if (lane == 0 && !gave_up) {
-            const Agg incl = agg_combine(excl, own, bits);
+            const Agg incl = agg_combine<F>(excl, own, bits);
             if (NX) put_x(D + 1 + NX, 2, incl);
             st_rel(&D[0], lb_status(2, epoch, incl.c));
+            if (below == 63) { // the group's inclusive value, beside its aggregate
+                if (NX) put_x(group(g) + 1 + NX, 2, incl);
+                st_rel(group(g), lb_status(2, epoch, incl.c));
+            }
         }
         return excl;
     }

@@ -126,14 +126,14 @@ __device__ inline uint64_t tile_window(const V* s_vals, uint32_t ctile, uint32_t
     return w;
 }
 
-// Look-back for tile t (one wave); the last tile writes the statistics and
-// advances the epoch.
-template <uint32_t F>
+// Look-back for tile t (one wave; PUBLISHED: its aggregate is already out);
+// the last tile writes the statistics and advances the epoch.
+template <uint32_t F, bool PUBLISHED = false>
 __device__ inline void tile_lookback(const CompressArgs& a, const Lookback<F>& L, uint64_t t, uint32_t ctile,
                                      uint32_t otile, uint64_t own_w, uint32_t lane, uint64_t* s_P, uint64_t* s_carry)
 {
     const Agg own{ctile, own_w, otile, 0};
-    const Agg ex = L.run(t, own, lane);
+    const Agg ex = PUBLISHED ? L.walk(t, own, lane) : L.run(t, own, lane);
     if (lane == 0) {
         *s_P = ex.c;
         if (s_carry) *s_carry = ex.w;
@@ -434,12 +434,14 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
         if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
         a.mask_out[mw0 + tid] = m;
     }
-    // container outputs: wave 0 looks back first (its words stay in registers)
+    // container outputs: wave 0 publishes the tile's aggregate before
+    // compacting its share, and looks back after
+    const uint32_t otile = (OVF && lead) ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
     if (OK == K_STRIDED && lead) {
         const Lookback<F> L = make_lookback<F>(a, s_epoch);
-        L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
-        const uint32_t otile = OVF ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
-        tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, nullptr);
+        const Agg own{ctile, 0, otile, 0};
+        L.publish_agg(t, own, lane);
+        if ((t & 63) == 63) L.publish_group_early(t, own, lane);
     }
     // compact the selected values into LDS at their tile ranks
     const uint32_t vm32 = (uint32_t)(a.in.mask & a.out.mask);
@@ -455,6 +457,11 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
                 if (__builtin_amdgcn_ubfe(gg, s, 1)) *dst++ = (V)((uint32_t)(x[r] >> sh) & vm32);
     }
     if (OK == K_STRIDED) {
+        if (lead) {
+            const Lookback<F> L = make_lookback<F>(a, s_epoch);
+            L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
+            tile_lookback<F, true>(a, L, t, ctile, otile, 0, lane, &s_P, nullptr);
+        }
         __syncthreads();
         const uint64_t P = s_P;
         switch (a.out.cbytes) {
@@ -476,7 +483,6 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
         if (lead) {
             const Lookback<F> L = make_lookback<F>(a, s_epoch);
             L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
-            const uint32_t otile = OVF ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
             const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
             tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
         }
