@@ -329,16 +329,17 @@ __host__ __device__ inline uint64_t* ws_gbpart(void* ws) { return (uint64_t*)((u
 //   [63:48] epoch&0xffff [47:46] level [45:0] payload
 // (window low 46 bits; window high 17 bits | overflow << 17; sum low 46
 // bits; sum high 18 bits). Per tile: status, aggregate words, inclusive
-// words; per group of 64 tiles the same, for the whole group. The group's
-// last tile publishes the group aggregate (as soon as its 63 group-mates'
-// aggregates are in, or during its own walk) and, with its own inclusive
-// value, the group's. A look-back reads, in one round trip, the tiles from
-// its predecessor down to the first tile of the previous group (so it
-// never waits for that group's words) together with the 64 groups older
-// than that (64 consecutive words), and stops at the nearest tile or group
-// whose inclusive value is known. Descriptor reads are device-coherent
-// loads that each move a whole memory line: the layout keeps a walk's lines
-// few (about a dozen).
+// words. Per group of 64 tiles: status and aggregate words of the whole
+// group, published by the group's last tile as soon as its 63 group-mates'
+// aggregates are in (or during its own walk); the group's inclusive value is
+// that tile's. A look-back reads, in one round trip, the tiles from its
+// predecessor down to the first tile of the previous group (so it never
+// waits for that group's words) together with, for the 64 groups older than
+// that, each group's aggregate and its last tile's status, and stops at the
+// nearest tile or group whose inclusive value is known. Descriptor reads are
+// device-coherent loads that each move a whole memory line; words that many
+// walks poll are written once per launch (the group aggregates), since
+// rewriting polled lines slows the memory path that serves them.
 //
 // Every wait is bounded: after LB_WAIT_TICKS of the 100 MHz wall clock the
 // waiting wave sets the workspace's abort word and all waiters stop. An
@@ -425,7 +426,7 @@ struct Lookback {
     static constexpr uint32_t NX = ((F & (LB_WIN | LB_OVF)) ? 2 : 0) + ((F & LB_SUM) ? 2 : 0); // payload words
     static constexpr uint32_t NXA = NX ? NX : 1;                                           // array size
     static constexpr uint32_t TS = 1 + 2 * NX; // tile stride: status, aggregate words, inclusive words
-    static constexpr uint32_t GS = 1 + 2 * NX; // group stride: status, aggregate words, inclusive words
+    static constexpr uint32_t GS = 1 + NX;     // group stride: status, aggregate words
     uint64_t* tiles;  // [cap][TS]
     uint64_t* groups; // [cap/64 + 2][GS]
     uint64_t cap;     // tiles
@@ -530,8 +531,8 @@ struct Lookback {
     // The last tile of a group (t = 64g+63), its aggregate published: once
     // its 63 group-mates have published theirs, publish the group's
     // aggregate, so later look-backs need not wait for this tile's own
-    // look-back. Skipped when a mate already holds an inclusive value (this
-    // tile's inclusive value then follows within one round trip).
+    // look-back. Skipped when a mate holds an inclusive value (only tile 0
+    // can: later walks then stop at this tile's inclusive value instead).
     __device__ void publish_group_early(uint64_t t, const Agg& own, uint32_t lane) const
     {
         const Agg zero{0, 0, 0, 0};
@@ -562,9 +563,12 @@ struct Lookback {
             st_rel(group(t >> 6), lb_status(1, epoch, ga.c));
         }
     }
-    // the look-back of a tile whose aggregate is published; publishes its
-    // inclusive value and returns the exclusive prefix (all lanes)
-    __device__ Agg walk(uint64_t t, const Agg& own, uint32_t lane) const
+    // The look-back of a tile whose aggregate is published; returns the
+    // exclusive prefix (all lanes). Only a group's last tile publishes its
+    // inclusive value (the one later walks stop at; the others would mostly
+    // arrive too late to shorten a walk and cost a second write into a line
+    // that walks are polling). group_out: the group aggregate is already out.
+    __device__ Agg walk(uint64_t t, const Agg& own, uint32_t lane, bool group_out = false) const
     {
         const Agg zero{0, 0, 0, 0};
         uint64_t* D = tile(t);
@@ -580,9 +584,10 @@ struct Lookback {
         // groups older than the window: lane j -> group g-2-j
         const int64_t h0 = (int64_t)g - 2 - (int64_t)lane;
         const uint64_t* gq = group(h0 >= 0 ? (uint64_t)h0 : 0);
+        const uint64_t* lt = tile(h0 >= 0 ? (uint64_t)h0 * 64 + 63 : 0);
         // aggregate payloads come with the statuses; the one inclusive
         // payload a walk uses (at its stop) is fetched once the stop is known
-        uint64_t sn = 0, sf = 0, gs = 0;
+        uint64_t sn = 0, sf = 0, gs = 0, ls = 0;
         uint64_t xan[NXA], xaf[NXA], xg[NXA];
         if (an) {
             sn = ld_acq(qn);
@@ -593,6 +598,7 @@ struct Lookback {
             load_x(qf + 1, xaf);
         }
         if (h0 >= 0) {
+            ls = ld_acq(lt);
             gs = ld_acq(gq);
             load_x(gq + 1, xg);
         }
@@ -644,7 +650,7 @@ struct Lookback {
         Agg excl = wave_combine_ordered<F>(vn, lane, bits);
         if (nw > 64 && stop >= 64) excl = agg_combine<F>(wave_combine_ordered<F>(vf, lane, bits), excl, bits);
         bool done = stop < nw || gave_up;
-        if (below == 63 && stop >= 63 && !gave_up) {
+        if (below == 63 && stop >= 63 && !gave_up && !group_out) {
             // group aggregate of group g: its other 63 tiles are near lanes 0..62
             const Agg ga = agg_combine<F>(wave_combine_ordered<F>(lane < 63 ? vn : zero, lane, bits), own, bits);
             if (lane == 0) {
@@ -658,7 +664,9 @@ struct Lookback {
             const bool act = h >= 0;
             if (gb != g - 1) {
                 gq = group(act ? (uint64_t)h : 0);
+                lt = tile(act ? (uint64_t)h * 64 + 63 : 0);
                 if (act) {
+                    ls = ld_acq(lt);
                     gs = ld_acq(gq);
                     load_x(gq + 1, xg);
                 }
@@ -667,17 +675,16 @@ struct Lookback {
             uint64_t t2 = 0, p2 = 0;
             Agg vp = zero, vg = zero;
             for (;;) {
-                // stop at the nearest group with its inclusive value
-                const bool gok = act && lb_status_ok(gs, epoch);
-                p2 = ballot(!act || (gok && (gs >> 62) == 2));
+                // stop at the nearest group whose last tile has its inclusive value
+                p2 = ballot(!act || (lb_status_ok(ls, epoch) && (ls >> 62) == 2));
                 gstop = p2 ? (uint32_t)__builtin_ctzll(p2) : 64;
                 bool hasP = false;
                 if (lane == gstop && act) {
                     uint64_t xl[NXA];
-                    load_x(gq + 1 + NX, xl);
-                    hasP = decode(xl, 2, gs & CNT_MASK, vp);
+                    load_x(lt + 1 + NX, xl);
+                    hasP = decode(xl, 2, ls & CNT_MASK, vp);
                 }
-                const bool hasG = gok && (gs >> 62) == 1 && decode(xg, 1, gs & CNT_MASK, vg);
+                const bool hasG = act && lb_status_ok(gs, epoch) && decode(xg, 1, gs & CNT_MASK, vg);
                 const bool need = (lane < gstop && !hasG) || (lane == gstop && act && !hasP);
                 if (!ballot(need)) break;
                 if (!keep_waiting(it2, t2)) {
@@ -685,6 +692,7 @@ struct Lookback {
                     break;
                 }
                 if (need && lane < gstop) {
+                    ls = ld_acq(lt);
                     gs = ld_acq(gq);
                     load_x(gq + 1, xg);
                 }
@@ -695,14 +703,10 @@ struct Lookback {
             done = p2 != 0 || gave_up;
             gb = gb >= 64 ? gb - 64 : 0;
         }
-        if (lane == 0 && !gave_up) {
+        if (lane == 0 && !gave_up && below == 63) {
             const Agg incl = agg_combine<F>(excl, own, bits);
             if (NX) put_x(D + 1 + NX, 2, incl);
             st_rel(&D[0], lb_status(2, epoch, incl.c));
-            if (below == 63) { // the group's inclusive value, beside its aggregate
-                if (NX) put_x(group(g) + 1 + NX, 2, incl);
-                st_rel(group(g), lb_status(2, epoch, incl.c));
-            }
         }
         return excl;
     }

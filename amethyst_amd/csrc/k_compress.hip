@@ -126,14 +126,15 @@ __device__ inline uint64_t tile_window(const V* s_vals, uint32_t ctile, uint32_t
     return w;
 }
 
-// Look-back for tile t (one wave; PUBLISHED: its aggregate is already out);
+// Look-back for tile t (one wave; PUBLISHED: its aggregate, and for a
+// group's last tile the group's, are already out);
 // the last tile writes the statistics and advances the epoch.
 template <uint32_t F, bool PUBLISHED = false>
 __device__ inline void tile_lookback(const CompressArgs& a, const Lookback<F>& L, uint64_t t, uint32_t ctile,
                                      uint32_t otile, uint64_t own_w, uint32_t lane, uint64_t* s_P, uint64_t* s_carry)
 {
     const Agg own{ctile, own_w, otile, 0};
-    const Agg ex = PUBLISHED ? L.walk(t, own, lane) : L.run(t, own, lane);
+    const Agg ex = PUBLISHED ? L.walk(t, own, lane, true) : L.run(t, own, lane);
     if (lane == 0) {
         *s_P = ex.c;
         if (s_carry) *s_carry = ex.w;
@@ -250,7 +251,6 @@ constexpr uint32_t CW_MASKW = 256; // output mask words per tile (4*R*eps)
 constexpr uint32_t CD_RMAX = 8;
 constexpr uint32_t CD_WAVES = 4;
 constexpr uint32_t CD_THREADS = CD_WAVES * 64;
-constexpr uint32_t CD_WORDS = CD_WAVES * 64 * CD_RMAX;  // input words per tile (max)
 constexpr uint32_t CD_STAGE = 20480;                    // LDS bytes: input words, then compacted values
 
 // Copy values s_vals[0, cnt) to out[P, P + cnt) (capacity-checked), four
@@ -349,8 +349,6 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             if (tid < NMW && mw0 + tid < nmw) m = a.mask_in[mw0 + tid];
             if (tid < NMW) s_mask[tid] = m;
         }
-        else if (a.mask_out && !pow2 && tid < NMW)
-            s_mask[tid] = 0;
         __builtin_amdgcn_s_waitcnt(0); // this wave's global->LDS copies
     }
     __syncthreads();
@@ -408,11 +406,23 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
                     const uint64_t mi = mw0 + (uint64_t)ci * eps + lane / S;
                     if ((lane & (S - 1)) == S - 1 && mi < nmw) a.mask_out[mi] = m;
                 }
-                else if (gg) {
-                    const uint32_t w = p >> 6, off = p & 63;
-                    atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
-                    if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
-                }
+                else // the selection bits, into this wave's own (consumed) input words
+                    ((uint32_t*)(s_x + wave * 64 * R))[r * 64 + lane] = gg;
+            }
+        }
+        if (PRED && a.mask_out && !pow2) {
+            // the wave's R*eps (<= 64) mask words: lane m gathers word m from
+            // the selection bits of the (at most ceil(64/eps)+1) words it spans
+            const uint32_t* sg = (const uint32_t*)(s_x + wave * 64 * R);
+            const uint32_t nwm = R * eps;
+            if (lane < nwm) {
+                const uint32_t wa = (lane * 64) / eps;       // the word holding the mask word's first element
+                const uint32_t sk = lane * 64 - wa * eps;    // its elements before that one
+                uint64_t m = sg[wa] >> sk;
+                for (uint32_t w = wa + 1, sh = eps - sk; sh < 64 && w < 64 * R; w++, sh += eps)
+                    m |= (uint64_t)sg[w] << sh;
+                const uint64_t mi = mw0 + (uint64_t)wave * nwm + lane;
+                if (mi < nmw) a.mask_out[mi] = m; // bits past n are zero (cleared above)
             }
         }
     }
@@ -429,7 +439,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     const uint32_t cc = lane < NC ? s_ccnt[lane] : 0;
     const uint32_t cinc = wave_incl_scan(cc);
     const uint32_t ctile = __builtin_amdgcn_readlane(cinc, 63);
-    if (a.mask_out && (!PRED || !pow2) && tid < NMW && mw0 + tid < nmw) {
+    if (!PRED && a.mask_out && tid < NMW && mw0 + tid < nmw) {
         uint64_t m = s_mask[tid];
         if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
         a.mask_out[mw0 + tid] = m;
