@@ -30,6 +30,8 @@
 //      no read-modify-write.
 // Descriptor words carry the launch epoch (bumped by the last tile), so the
 // workspace needs no clearing between launches or graph replays.
+#include <type_traits>
+
 #include "am_device.hpp"
 #include "am_host.hpp"
 
@@ -252,6 +254,7 @@ constexpr uint32_t CD_RMAX = 8;
 constexpr uint32_t CD_WAVES = 4;
 constexpr uint32_t CD_THREADS = CD_WAVES * 64;
 constexpr uint32_t CD_STAGE = 20480;                    // LDS bytes: input words, then compacted values
+constexpr uint32_t CD_STAGE_C = 16384; // container outputs: the same, compacted values in passes (8 blocks per CU)
 
 // Copy values s_vals[0, cnt) to out[P, P + cnt) (capacity-checked), four
 // per thread step with one vector store where the output is aligned.
@@ -292,14 +295,22 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
         if (P + i < cap) out[P + i] = (C)s_vals[i];
 }
 
+// container outputs of a count-only scan fit 8 waves per SIMD (8 blocks per
+// CU, the most 4-wave blocks a CU holds): ask the compiler to keep them there
+template <int OK, bool OVF>
+struct CdWaves {
+    static constexpr int V = (OK == K_STRIDED && !OVF) ? 8 : 1;
+};
+
 template <int OK, bool PRED, bool OVF, int VB, bool WIDE>
-__global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
+__global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_direct(CompressArgs a)
 {
     constexpr uint32_t F = CsFlags<OK, OVF>::F;
     using V = typename VT<VB>::T;
     typedef __attribute__((address_space(3))) void* lds_ptr;
-    __shared__ uint64_t s_buf[CD_STAGE / 8]; // input words, then compacted values
-    __shared__ uint64_t s_mask[CW_MASKW];
+    constexpr uint32_t STAGE = OK == K_STRIDED ? CD_STAGE_C : CD_STAGE;
+    __shared__ uint64_t s_buf[STAGE / 8]; // input words, then compacted values
+    __shared__ uint64_t s_mask[PRED ? 1 : CW_MASKW]; // input mask words
     __shared__ uint32_t s_ccnt[4 * CD_RMAX], s_covf[4 * CD_RMAX];
     __shared__ uint64_t s_P, s_carry;
     __shared__ uint32_t s_epoch, s_abort;
@@ -358,6 +369,27 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
     uint32_t g[CD_RMAX], lp[CD_RMAX];
     const uint32_t om32 = (uint32_t)a.out.mask;
     const uint32_t leps = __umul24(lane, eps);
+    // selection bits of the lane's word in chunk ci (word xr)
+    auto select_word = [&](uint32_t ci, uint64_t xr) -> uint32_t {
+        uint32_t gg;
+        if (!PRED) {
+            const uint32_t p = ci * 64 * eps + leps; // first bit of the word's elements in the tile mask
+            const uint32_t w = p >> 6, off = p & 63;
+            uint64_t m = s_mask[w] >> off;
+            if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
+            gg = (uint32_t)m & (uint32_t)lowmask(eps);
+        }
+        else if (WIDE)
+            gg = a.pr(xr & a.in.mask);
+        else
+            gg = select_fields(xr, eps, fw, bw, a.tp);
+        if (last) { // elements past n (only in the last tile)
+            const uint64_t e0 = (w0 + ci * 64 + lane) * eps;
+            const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
+            gg &= (uint32_t)lowmask(nv);
+        }
+        return gg;
+    };
     {
 #pragma unroll
         for (uint32_t r = 0; r < CD_RMAX; r++) {
@@ -366,25 +398,9 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
             lp[r] = 0;
             if (r >= R) continue;
             const uint32_t ci = wave * R + r;
-            const uint32_t wl = ci * 64 + lane;
-            x[r] = s_x[wl];
-            const uint32_t p = ci * 64 * eps + leps; // first bit of the word's elements in the tile mask
-            uint32_t gg = 0, go = 0;
-            if (!PRED) {
-                const uint32_t w = p >> 6, off = p & 63;
-                uint64_t m = s_mask[w] >> off;
-                if (off && off + eps > 64) m |= s_mask[w + 1] << (64 - off);
-                gg = (uint32_t)m & (uint32_t)lowmask(eps);
-            }
-            else if (WIDE)
-                gg = a.pr(x[r] & a.in.mask);
-            else
-                gg = select_fields(x[r], eps, fw, bw, a.tp);
-            if (last) { // elements past n (only in the last tile)
-                const uint64_t e0 = (w0 + wl) * eps;
-                const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
-                gg &= (uint32_t)lowmask(nv);
-            }
+            x[r] = s_x[ci * 64 + lane];
+            uint32_t go = 0;
+            const uint32_t gg = select_word(ci, x[r]);
             if (OVF) {
                 if (WIDE) go = (gg & 1) && ((x[r] & a.in.mask) & ~a.out.mask) != 0;
                 else
@@ -453,19 +469,32 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
         L.publish_agg(t, own, lane);
         if ((t & 63) == 63) L.publish_group_early(t, own, lane);
     }
-    // compact the selected values into LDS at their tile ranks
+    // compact the selected values into LDS at their tile ranks; container
+    // outputs stage CAP values per pass (one pass unless the tile selects
+    // more), packed outputs always fit
+    constexpr uint32_t CAP = STAGE / sizeof(V);
     const uint32_t vm32 = (uint32_t)(a.in.mask & a.out.mask);
+    auto stage = [&](uint32_t base, auto checked) {
+        constexpr bool CHK = decltype(checked)::value;
 #pragma unroll
-    for (uint32_t r = 0; r < CD_RMAX; r++) {
-        if (r >= R || !g[r]) continue;
-        const uint32_t ci = wave * R + r;
-        V* dst = s_vals + (__builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r]);
-        const uint32_t gg = g[r];
-        if (WIDE) *dst = (V)(x[r] & a.in.mask & a.out.mask);
-        else
-            for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
-                if (__builtin_amdgcn_ubfe(gg, s, 1)) *dst++ = (V)((uint32_t)(x[r] >> sh) & vm32);
-    }
+        for (uint32_t r = 0; r < CD_RMAX; r++) {
+            if (r >= R || !g[r]) continue;
+            const uint32_t ci = wave * R + r;
+            uint32_t d = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r] - base;
+            const uint32_t gg = g[r];
+            if (WIDE) {
+                if (!CHK || d < CAP) s_vals[d] = (V)(x[r] & a.in.mask & a.out.mask);
+            }
+            else
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                    if (__builtin_amdgcn_ubfe(gg, s, 1)) {
+                        if (!CHK || d < CAP) s_vals[d] = (V)((uint32_t)(x[r] >> sh) & vm32);
+                        d++;
+                    }
+        }
+    };
+    if (OK != K_STRIDED || ctile <= CAP) stage(0u, std::false_type{});
+    else stage(0u, std::true_type{});
     if (OK == K_STRIDED) {
         if (lead) {
             const Lookback<F> L = make_lookback<F>(a, s_epoch);
@@ -474,11 +503,38 @@ __global__ __launch_bounds__(CD_THREADS) void k_compress_direct(CompressArgs a)
         }
         __syncthreads();
         const uint64_t P = s_P;
-        switch (a.out.cbytes) {
-            case 1: copy_out<uint8_t>(a, s_vals, P, ctile, tid); break;
-            case 2: copy_out<uint16_t>(a, s_vals, P, ctile, tid); break;
-            case 4: copy_out<uint32_t>(a, s_vals, P, ctile, tid); break;
-            default: copy_out<uint64_t>(a, s_vals, P, ctile, tid);
+        for (uint32_t base = 0;;) {
+            const uint32_t cnt = ctile - base < CAP ? ctile - base : CAP;
+            switch (a.out.cbytes) {
+                case 1: copy_out<uint8_t>(a, s_vals, P + base, cnt, tid); break;
+                case 2: copy_out<uint16_t>(a, s_vals, P + base, cnt, tid); break;
+                case 4: copy_out<uint32_t>(a, s_vals, P + base, cnt, tid); break;
+                default: copy_out<uint64_t>(a, s_vals, P + base, cnt, tid);
+            }
+            base += CAP;
+            if (base >= ctile) break;
+            __syncthreads(); // the pass is out: stage the next
+            // (chunk by chunk from the tile's words again: the words, bits
+            // and ranks are not kept live across the look-back, which would
+            // cost occupancy)
+            for (uint32_t r = 0; r < R; r++) {
+                const uint32_t ci = wave * R + r;
+                const uint64_t gw = w0 + ci * 64 + lane;
+                const uint64_t xr = gw < nw ? a.in.p[gw] : 0;
+                const uint32_t gg = select_word(ci, xr);
+                const uint32_t c = (uint32_t)__popc(gg);
+                uint32_t d = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + wave_incl_scan(c) - c - base;
+                if (WIDE) {
+                    if (gg && d < CAP) s_vals[d] = (V)(xr & a.in.mask & a.out.mask);
+                }
+                else
+                    for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                        if (__builtin_amdgcn_ubfe(gg, s, 1)) {
+                            if (d < CAP) s_vals[d] = (V)((uint32_t)(xr >> sh) & vm32);
+                            d++;
+                        }
+            }
+            __syncthreads();
         }
         if (last && tid == 0) {
             uint64_t K = P + ctile;
@@ -698,11 +754,12 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
         a.fw = ci.kind == K_SLAB ? ci.bits : 8 * ci.cbytes;
         a.eps = 64 / a.fw;
         {
-            // mask words per tile 4*R*eps <= 256; compacted values 256*R*eps*VB <= CD_STAGE
+            // mask words per tile 4*R*eps <= 256; packed outputs: compacted values 256*R*eps*VB <= CD_STAGE
             const uint32_t vbh = value_bytes(ci.bits < co.bits ? ci.bits : co.bits);
             a.R = 64 / a.eps;
             const uint32_t rs = CD_STAGE / (CD_WAVES * 64 * a.eps * vbh);
-            if (a.R > rs) a.R = rs;
+            if (co.kind != K_STRIDED && a.R > rs) a.R = rs; // packed outputs stage a whole tile
+            static_assert(CD_STAGE_C >= CD_WAVES * 64 * CD_RMAX * 8, "container tile words fit the stage");
             if (a.R > CD_RMAX) a.R = CD_RMAX;
             if (a.R < 1) a.R = 1;
             a.lds16 = ((uintptr_t)ci.p & 15) == 0;
