@@ -332,11 +332,13 @@ __host__ __device__ inline uint64_t* ws_gbpart(void* ws) { return (uint64_t*)((u
 // words. Per group of 64 tiles: status and aggregate words of the whole
 // group, published by the group's last tile as soon as its 63 group-mates'
 // aggregates are in (or during its own walk); the group's inclusive value is
-// that tile's. A look-back reads, in one round trip, the tiles from its
-// predecessor down to the first tile of the previous group (so it never
-// waits for that group's words) together with, for the 64 groups older than
-// that, each group's aggregate and its last tile's status, and stops at the
-// nearest tile or group whose inclusive value is known. Descriptor reads are
+// that tile's. A look-back reads the tiles from its predecessor down to the
+// first tile of the previous group (so it never waits for that group's
+// words) and, for the 64 groups older than that, each group's aggregate and
+// its last tile's status -- in one round trip for a count-only scan, in two
+// when payload words come along (holding both sets would cost occupancy) --
+// and stops at the nearest tile or group whose inclusive value is known.
+// Descriptor reads are
 // device-coherent loads that each move a whole memory line; words that many
 // walks poll are written once per launch (the group aggregates), since
 // rewriting polled lines slows the memory path that serves them.
@@ -597,7 +599,7 @@ struct Lookback {
             sf = ld_acq(qf);
             load_x(qf + 1, xaf);
         }
-        if (h0 >= 0) {
+        if (NX == 0 && h0 >= 0) { // (with payload words, holding these too costs occupancy)
             ls = ld_acq(lt);
             gs = ld_acq(gq);
             load_x(gq + 1, xg);
@@ -658,11 +660,12 @@ struct Lookback {
                 st_rel(group(g), lb_status(1, epoch, ga.c));
             }
         }
-        // whole groups, 64 per step (lane j -> group gb-1-j); the first step's words are in hand
+        // whole groups, 64 per step (lane j -> group gb-1-j); a count-only
+        // scan has the first step's words in hand
         for (uint64_t gb = g - 1; !done;) {
             const int64_t h = (int64_t)gb - 1 - (int64_t)lane;
             const bool act = h >= 0;
-            if (gb != g - 1) {
+            if (NX != 0 || gb != g - 1) {
                 gq = group(act ? (uint64_t)h : 0);
                 lt = tile(act ? (uint64_t)h * 64 + 63 : 0);
                 if (act) {

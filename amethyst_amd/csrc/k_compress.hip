@@ -172,49 +172,68 @@ __device__ inline void tile_write(const CompressArgs& a, const V* s_vals, uint64
         }
     }
     else if (OK == K_SLAB) {
-        const uint64_t eps = a.out.eps;
+        // word kf + j holds tile values [j*eps - q0, (j+1)*eps - q0); values
+        // before the tile come from the carried window. Tile-relative
+        // indices fit 32 bits (a tile holds < 2^17 values).
+        const uint32_t eps = a.out.eps;
         const uint64_t kf = P / eps;
+        const int32_t q0 = (int32_t)(P - kf * eps); // carried values in the first word
         uint64_t kl = (P + ctile) / eps;
         if (last && ((P + ctile) % eps)) kl += 1;
         const uint64_t wcap = (ocap + eps - 1) / eps;
         if (kl > wcap) kl = wcap;
-        for (uint64_t k = kf + tid; k < kl; k += CS_THREADS) {
+        const uint32_t nk = kl > kf ? (uint32_t)(kl - kf) : 0;
+        // values at or past min(ctile, ocap - P) stay zero
+        const int64_t room = (int64_t)ocap - (int64_t)P;
+        const int32_t lim = room < (int64_t)ctile ? (int32_t)(room < -(int64_t)eps ? -(int64_t)eps : room) : (int32_t)ctile;
+        for (uint32_t j = tid; j < nk; j += CS_THREADS) {
+            const int32_t i0 = (int32_t)(j * eps) - q0;
             uint64_t word = 0;
-            const uint64_t e0 = k * eps;
-            for (uint32_t s = 0; s < eps; s++) {
-                const uint64_t e = e0 + s;
-                if (e >= ocap) break;
-                uint64_t v;
-                if (e < P) v = (carry >> (63 - (P - e) * bo)) & om;
-                else if (e < P + ctile) v = s_vals[e - P];
-                else break;
-                word |= v << (s * bo);
+            if (i0 >= 0 && i0 + (int32_t)eps <= lim) { // whole word from the staged values
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += bo) word |= (uint64_t)s_vals[i0 + s] << sh;
             }
-            a.out.p[k] = word;
+            else {
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += bo) {
+                    const int32_t i = i0 + (int32_t)s;
+                    if (i >= lim) break;
+                    const uint64_t v = i < 0 ? (carry >> (63 - (uint32_t)(-i) * bo)) & om : (uint64_t)s_vals[i];
+                    word |= v << sh;
+                }
+            }
+            a.out.p[kf + j] = word;
         }
     }
     else {
+        // word kf + j covers tile stream bits [j*64 - o0, j*64 - o0 + 64);
+        // bits before the tile come from the carried window
         const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
         const uint64_t kf = B0 >> 6;
+        const int32_t o0 = (int32_t)(B0 & 63);
         uint64_t kl = B1 >> 6;
         if (last && (B1 & 63)) kl += 1;
         const uint64_t wcap = (Bcap + 63) >> 6;
         if (kl > wcap) kl = wcap;
-        for (uint64_t k = kf + tid; k < kl; k += CS_THREADS) {
-            const uint64_t b0 = k * 64;
+        const uint32_t nk = kl > kf ? (uint32_t)(kl - kf) : 0;
+        // x / bo = (x * (floor(2^32/bo) + 1)) >> 32, exact for x < 2^24 (the
+        // multiplier needs 33 bits when bo = 1)
+        const uint64_t bmagic = (1ull << 32) / bo + 1;
+        auto div_bo = [&](uint32_t x) { return (uint32_t)(((uint64_t)x * bmagic) >> 32); };
+        for (uint32_t j = tid; j < nk; j += CS_THREADS) {
+            const int32_t rb = (int32_t)(j * 64) - o0; // tile stream bit of the word's bit 0
             uint64_t word = 0;
-            if (b0 < B0) word = carry >> (b0 + 63 - B0);
-            uint64_t e0 = b0 / bo;
-            if (e0 < P) e0 = P;
-            uint64_t e1 = (b0 + 63) / bo + 1;
-            if (e1 > P + ctile) e1 = P + ctile;
-            for (uint64_t e = e0; e < e1; e++) {
-                const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
-                const uint64_t v = s_vals[e - P];
+            uint32_t i = 0;
+            if (rb < 0) word = carry >> (63 + rb); // carry bit 62 = stream bit B0 - 1
+            else i = div_bo((uint32_t)rb);
+            uint32_t ie = div_bo((uint32_t)(rb + 63 + (int32_t)bo)); // one past the last value touching the word
+            if (ie > ctile) ie = ctile;
+            for (; i < ie; i++) {
+                const int32_t pos = (int32_t)(i * bo) - rb;
+                const uint64_t v = s_vals[i];
                 word |= pos >= 0 ? (v << pos) : (v >> (-pos));
             }
+            const uint64_t b0 = (kf + j) * 64;
             if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-            a.out.p[k] = word;
+            a.out.p[kf + j] = word;
         }
     }
 }
@@ -312,7 +331,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
     __shared__ uint64_t s_buf[STAGE / 8]; // input words, then compacted values
     __shared__ uint64_t s_mask[PRED ? 1 : CW_MASKW]; // input mask words
     __shared__ uint32_t s_ccnt[4 * CD_RMAX], s_covf[4 * CD_RMAX];
-    __shared__ uint64_t s_P, s_carry;
+    __shared__ uint64_t s_P, s_carry, s_win[CD_WAVES];
     __shared__ uint32_t s_epoch, s_abort;
     uint64_t* const s_x = s_buf;
     V* const s_vals = (V*)s_buf;
@@ -460,12 +479,47 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
         a.mask_out[mw0 + tid] = m;
     }
-    // container outputs: wave 0 publishes the tile's aggregate before
-    // compacting its share, and looks back after
+    // wave 0 publishes the tile's aggregate before the block compacts, and
+    // looks back after. Packed outputs carry the tile's window (the last 63
+    // bits of its output stream): each wave ORs in its values among the
+    // last ceil(63/bits) ranks, taken from registers (only chunks reaching
+    // into that tail do any work).
     const uint32_t otile = (OVF && lead) ? wave_sum32(lane < NC ? s_covf[lane] : 0) : 0;
-    if (OK == K_STRIDED && lead) {
+    uint64_t own_w = 0;
+    if (OK != K_STRIDED) {
+        const uint32_t bo = a.out.bits;
+        uint64_t wp = 0;
+        if (bo < 64) {
+            const uint32_t kq = (63 + bo - 1) / bo;
+            const uint32_t q0 = ctile > kq ? ctile - kq : 0;
+            const int32_t org = (int32_t)(ctile * bo) - 63; // stream bit of window bit 0
+            const uint32_t vmw = (uint32_t)(a.in.mask & a.out.mask);
+#pragma unroll
+            for (uint32_t r = 0; r < CD_RMAX; r++) {
+                if (r >= R) continue;
+                const uint32_t ci = wave * R + r;
+                if ((uint32_t)__builtin_amdgcn_readlane(cinc, ci) <= q0) continue; // chunk ends before the tail
+                uint32_t q = __builtin_amdgcn_readlane(cinc, ci) - __builtin_amdgcn_readlane(cc, ci) + lp[r];
+                const uint32_t gg = g[r];
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
+                    if (__builtin_amdgcn_ubfe(gg, s, 1)) {
+                        if (q >= q0) {
+                            const uint64_t v = WIDE ? (x[r] & a.in.mask & a.out.mask) : (uint64_t)((uint32_t)(x[r] >> sh) & vmw);
+                            const int32_t pos = (int32_t)(q * bo) - org;
+                            wp |= pos >= 0 ? v << pos : v >> (-pos);
+                        }
+                        q++;
+                    }
+            }
+            wp = wave_or_u64(wp);
+        }
+        if (lane == 0) s_win[wave] = wp;
+        __syncthreads();
+        own_w = (s_win[0] | s_win[1] | s_win[2] | s_win[3]) & WIN_BITS;
+    }
+    if (lead) {
         const Lookback<F> L = make_lookback<F>(a, s_epoch);
-        const Agg own{ctile, 0, otile, 0};
+        const Agg own{ctile, own_w, otile, 0};
         L.publish_agg(t, own, lane);
         if ((t & 63) == 63) L.publish_group_early(t, own, lane);
     }
@@ -544,13 +598,13 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         }
     }
     else {
-        // packed outputs: the tile's window needs its staged values
+        // packed outputs: output words are assembled from the staged values
+        // and the carried window of the predecessors
         __syncthreads();
         if (lead) {
             const Lookback<F> L = make_lookback<F>(a, s_epoch);
             L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
-            const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
-            tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
+            tile_lookback<F, true>(a, L, t, ctile, otile, own_w, lane, &s_P, &s_carry);
         }
         __syncthreads();
         tile_write<OK>(a, s_vals, s_P, s_carry, ctile, last, tid);
