@@ -523,6 +523,10 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         L.publish_agg(t, own, lane);
         if ((t & 63) == 63) L.publish_group_early(t, own, lane);
     }
+    // (every LDS operation so far has completed; saying so keeps the compiler
+    // from re-waiting on them inside the staging loops below, where each such
+    // wait would also drain the previous value store)
+    __builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0)
     // compact the selected values into LDS at their tile ranks; container
     // outputs stage CAP values per pass (one pass unless the tile selects
     // more), packed outputs always fit
@@ -539,7 +543,8 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
             if (WIDE) {
                 if (!CHK || d < CAP) s_vals[d] = (V)(x[r] & a.in.mask & a.out.mask);
             }
-            else
+            else // (a branch-free form storing every field, unselected ones to a
+                 // trash slot, measured slower: the extra LDS stores cost more)
                 for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
                     if (__builtin_amdgcn_ubfe(gg, s, 1)) {
                         if (!CHK || d < CAP) s_vals[d] = (V)((uint32_t)(x[r] >> sh) & vm32);
