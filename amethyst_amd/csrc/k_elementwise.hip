@@ -176,11 +176,111 @@ static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s
     return true;
 }
 
+// Word-per-lane binary op for bitstream inputs (in-width wi <= 32) into a
+// bitstream or slab output no narrower than the input (so a block's output
+// words need at most as many input words). The block copies its contiguous
+// input range of both columns into LDS with coalesced loads; each lane then
+// builds whole output words, taking each element's wi bits with one funnel
+// shift (v_alignbit) of two LDS dwords. Indices are block-relative 32-bit;
+// divisions by the out-width are a reciprocal multiply (exact below 2^24).
+constexpr uint32_t BS_WORDS = 1024;                // output words per block (4 per lane)
+constexpr uint32_t BS_IN = BS_WORDS + 4;           // input words per column staged per block
+struct BinStream {
+    Col a, b, out;
+    uint32_t wi, wo, eps_o, magic_o; // magic_o: floor(2^32/wo)+1 (64-bit multiply)
+    uint64_t words_i, words_o;
+};
+
+template <int OP, bool OSLAB>
+__global__ __launch_bounds__(256) void k_binary_stream(BinStream g)
+{
+    __shared__ uint64_t s_a[BS_IN + 1], s_b[BS_IN + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t n = g.out.n;
+    const uint32_t wi = g.wi, wo = g.wo;
+    const uint64_t q0 = (uint64_t)blockIdx.x * BS_WORDS;
+    const uint64_t qend = q0 + BS_WORDS < g.words_o ? q0 + BS_WORDS : g.words_o;
+    // the block's first element, and one past the last element its words hold
+    const uint64_t t0 = OSLAB ? q0 * g.eps_o : (q0 * 64) / wo;
+    uint64_t tend = OSLAB ? qend * g.eps_o : (qend * 64 - 1) / wo + 1;
+    if (tend > n) tend = n;
+    const uint64_t ib0 = (t0 * wi) >> 6;
+    uint64_t ib1 = (tend * wi + 63) >> 6;
+    if (ib1 > g.words_i) ib1 = g.words_i;
+    const uint32_t nin = (uint32_t)(ib1 - ib0); // <= BS_IN (wo >= wi)
+    for (uint32_t i = tid; i <= nin; i += 256) {
+        s_a[i] = i < nin ? g.a.p[ib0 + i] : 0;
+        s_b[i] = i < nin ? g.b.p[ib0 + i] : 0;
+    }
+    __syncthreads();
+    const uint32_t* sa = (const uint32_t*)s_a;
+    const uint32_t* sb = (const uint32_t*)s_b;
+    const uint32_t rb = (uint32_t)(t0 * wi - ib0 * 64);      // input bit of element t0 in the staged run
+    const uint32_t base = OSLAB ? 0 : (uint32_t)(q0 * 64 - t0 * wo); // output bit of word q0 past element t0's start
+    const uint32_t ne = (uint32_t)(tend - t0);                // elements the block may touch
+    const uint32_t im = (uint32_t)g.a.mask;
+    const uint64_t om = g.out.mask;
+    auto div_o = [&](uint32_t x) { return (uint32_t)(((uint64_t)x * g.magic_o) >> 32); };
+    for (uint32_t k = 0; k < BS_WORDS / 256; k++) {
+        const uint32_t qr = k * 256 + tid;
+        if (q0 + qr >= qend) break;
+        uint32_t e, e1;   // block-relative elements of the word, [e, e1)
+        int32_t pos;      // output bit of element e relative to the word
+        if (OSLAB) {
+            e = qr * g.eps_o;
+            e1 = e + g.eps_o;
+            pos = 0;
+        }
+        else {
+            const uint32_t ob = qr * 64 + base; // the word's first output bit past element t0's start
+            e = div_o(ob);
+            e1 = div_o(ob + 63) + 1;
+            pos = (int32_t)(e * wo) - (int32_t)ob;
+        }
+        if (e1 > ne) e1 = ne;
+        uint64_t word = 0;
+        for (; e < e1; e++, pos += (int32_t)wo) {
+            const uint32_t ib = rb + e * wi, j = ib >> 5, sh = ib & 31;
+            const uint32_t va = __builtin_amdgcn_alignbit(sa[j + 1], sa[j], sh) & im;
+            const uint32_t vb = __builtin_amdgcn_alignbit(sb[j + 1], sb[j], sh) & im;
+            const uint64_t r = binop<OP>((uint64_t)va, (uint64_t)vb) & om;
+            word |= pos >= 0 ? (r << pos) : (r >> (-pos));
+        }
+        g.out.p[q0 + qr] = word;
+    }
+}
+
+template <int OP>
+static bool launch_stream(const Col& a, const Col& b, const Col& o, hipStream_t s)
+{
+    if (a.kind != K_BITSTREAM || a.bits > 32 || o.bits < a.bits) return false;
+    if (o.kind != K_BITSTREAM && o.kind != K_SLAB) return false;
+    BinStream g{};
+    g.a = a;
+    g.b = b;
+    g.out = o;
+    g.wi = a.bits;
+    g.wo = o.bits;
+    g.eps_o = o.kind == K_SLAB ? 64 / o.bits : 0;
+    g.magic_o = (uint32_t)0; // unused for slab outputs
+    g.words_i = a.words;
+    g.words_o = o.words;
+    const uint64_t mo = (1ull << 32) / o.bits + 1;
+    if (mo >> 32) return false; // out-width 1 (only possible with in-width 1)
+    g.magic_o = (uint32_t)mo;
+    const dim3 grid((uint32_t)((g.words_o + BS_WORDS - 1) / BS_WORDS));
+    if (o.kind == K_SLAB) hipLaunchKernelGGL((k_binary_stream<OP, true>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_binary_stream<OP, false>), grid, dim3(256), 0, s, g);
+    return true;
+}
+
 template <int OP>
 static void launch_ew(const Col& a, const Col& b, const Col& o, hipStream_t s)
 {
-    if constexpr (OP != OP_COPY)
+    if constexpr (OP != OP_COPY) {
         if (launch_words<OP>(a, b, o, s)) return;
+        if (launch_stream<OP>(a, b, o, s)) return;
+    }
     uint64_t units = o.kind == K_STRIDED ? o.n : (o.kind == K_SLAB ? (o.n + o.eps - 1) / o.eps : (o.n * o.bits + 63) / 64);
     dim3 grid((uint32_t)((units + EW_UNITS - 1) / EW_UNITS));
 #define EW_CASE(AK, OK) hipLaunchKernelGGL((k_elementwise<AK, OK, OP>), grid, dim3(256), 0, s, a, b, o)

@@ -256,6 +256,27 @@ def test_binary_ops(am, oracle_mod):
                     assert np.array_equal(out.host_words(), exp), (bits, fmt, op, ofmt)
 
 
+def test_binary_bitstream_multiblock(am, oracle_mod):
+    """Bitstream inputs over many blocks of the word-per-lane stream kernel
+    (block-relative element and bit offsets, words straddling blocks), into
+    bitstream and slab outputs."""
+    rng = np.random.default_rng(17)
+    for bits in (2, 3, 11, 17, 31, 32):
+        n = 200003
+        a = rand_vals(rng, n, bits)
+        b = rand_vals(rng, n, bits)
+        ca = upload(am, oracle_mod, a, bits, 2)
+        cb = upload(am, oracle_mod, b, bits, 2)
+        for op in (0, 1, 7, 9, 10):  # add, mul, and, xor, sub
+            for ofmt in (2, 1):
+                ob = oracle_mod.outbits(op, bits)
+                say("binary-stream", bits, op, ofmt, ob)
+                out = am.binary(op, ca, cb, out_bits=ob, out_fmt=ofmt)
+                exp = oracle_mod.binary(op, 2, bits, oracle_mod.pack(2, bits, a), oracle_mod.pack(2, bits, b),
+                                        n, ofmt, ob)
+                assert np.array_equal(out.host_words(), exp), (bits, op, ofmt)
+
+
 def test_sum(am, oracle_mod):
     rng = np.random.default_rng(8)
     ws = am.Workspace(0)
