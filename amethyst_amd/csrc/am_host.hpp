@@ -46,7 +46,13 @@ inline int make_col(const am_column* c, Col* out)
         case AM_FMT_TIGHT: d.kind = K_STRIDED; d.cbytes = tight_bytes(c->bits); break;
         case AM_FMT_U32: d.kind = K_STRIDED; d.cbytes = 4; break;
         case AM_FMT_SLAB: d.kind = K_SLAB; d.cbytes = 8; break;
-        default: d.kind = K_BITSTREAM; d.cbytes = 8; break;
+        default:
+            // a bitstream whose width divides 64 (2..64) is byte for byte the
+            // slab of that width (no straddles, no dead bits): the word kernels
+            // take it; 1-bit columns stay streams
+            d.kind = (c->bits >= 2 && 64 % c->bits == 0) ? K_SLAB : K_BITSTREAM;
+            d.cbytes = 8;
+            break;
     }
     *out = d;
     return AM_OK;

@@ -72,6 +72,44 @@ __global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_
 }
 
 
+// Bitstream inputs of <= 32 bits: a block owns 4096 elements; it copies their
+// input words into LDS with coalesced loads, then lane l of a wave takes
+// element 64j + l with one funnel shift (v_alignbit) of two LDS dwords, and
+// the wave's ballot of the predicate is mask word j. One tile per block.
+constexpr uint32_t FS_J = 16;                   // mask words per wave
+constexpr uint32_t FS_TILE = 4 * FS_J * 64;     // elements per block
+constexpr uint32_t FS_IN = FS_TILE * 32 / 64 + 3; // staged input words (bits <= 32)
+
+__global__ __launch_bounds__(256) void k_filter_stream(Col in, Pred pr, uint64_t* mask_out)
+{
+    __shared__ uint64_t s_w[FS_IN];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t n = in.n, t0 = (uint64_t)blockIdx.x * FS_TILE;
+    const uint32_t w = in.bits;
+    const uint64_t ib0 = (t0 * w) >> 6;
+    const uint64_t tend = t0 + FS_TILE < n ? t0 + FS_TILE : n;
+    uint64_t ib1 = (tend * w + 63) >> 6;
+    if (ib1 > in.words) ib1 = in.words;
+    const uint32_t nin = (uint32_t)(ib1 - ib0);
+    for (uint32_t i = tid; i < FS_IN; i += 256) s_w[i] = i < nin ? in.p[ib0 + i] : 0;
+    __syncthreads();
+    const uint32_t* s32 = (const uint32_t*)s_w;
+    const uint32_t rb = (uint32_t)(t0 * w - ib0 * 64); // input bit of element t0 in the staged run
+    const uint32_t ne = (uint32_t)(tend - t0);
+    const uint32_t vm = (uint32_t)in.mask;
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < FS_J; j++) {
+        const uint32_t er = wave * (FS_J * 64) + j * 64 + lane;
+        const uint32_t b = rb + er * w; // < 2^18: its dword pair lies in s_w
+        const uint32_t v = __builtin_amdgcn_alignbit(s32[(b >> 5) + 1], s32[b >> 5], b & 31) & vm;
+        const uint64_t m = ballot(er < ne && pr.sel32(v));
+        if (lane == j) mine = m;
+    }
+    const uint64_t mw = (t0 >> 6) + wave * FS_J + lane;
+    if (lane < FS_J && mw < ((n + 63) >> 6)) mask_out[mw] = mine;
+}
+
 // Word-per-lane filter for slab and container inputs: a block owns a tile
 // of 256*R input words; each lane issues its R word loads at once and forms
 // each word's selection bits with select_fields (4 VALU per field). Mask
@@ -268,28 +306,37 @@ __global__ __launch_bounds__(256) void k_sum_partial(Col in, uint64_t* partial)
     }
     else if (IK == K_BITSTREAM && !WIDE) {
         // lane owns the elements that START in its word; a straddler takes
-        // its top bits from the next word
-        const uint64_t words = (in.n * in.bits + 63) >> 6;
-        for (uint64_t q0 = (uint64_t)blockIdx.x * 256 * SUM_UNROLL + threadIdx.x; q0 < words; q0 += gstride) {
+        // its top bits from the next word. Offsets are 32-bit: per block and
+        // step, (64*q0) mod bits once (scalar), then each lane's residue by
+        // a reciprocal multiply; fields (<= 32 bits) by one funnel shift.
+        const uint32_t w = in.bits;
+        const uint64_t words = (in.n * w + 63) >> 6;
+        const uint64_t nbits = in.n * w;
+        const uint64_t wmagic = (1ull << 32) / w + 1; // exact below 2^24
+        const uint32_t vm = (uint32_t)in.mask;
+        for (uint64_t qb = (uint64_t)blockIdx.x * 256 * SUM_UNROLL; qb < words; qb += gstride) {
+            const uint32_t r0 = (uint32_t)((qb * 64) % w); // bits of an element before word qb's bit 0, mod w
             uint64_t wv[SUM_UNROLL], nv[SUM_UNROLL];
 #pragma unroll
             for (uint32_t u = 0; u < SUM_UNROLL; u++) {
-                const uint64_t q = q0 + u * 256;
+                const uint64_t q = qb + u * 256 + threadIdx.x;
                 wv[u] = q < words ? in.p[q] : 0;
                 nv[u] = q + 1 < words ? in.p[q + 1] : 0;
             }
 #pragma unroll
             for (uint32_t u = 0; u < SUM_UNROLL; u++) {
-                const uint64_t q = q0 + u * 256;
+                const uint32_t d = u * 256 + threadIdx.x;
+                const uint64_t q = qb + d;
                 if (q >= words) continue;
-                const uint64_t b0 = q * 64;
-                uint64_t e = (b0 + in.bits - 1) / in.bits; // first element starting in this word
-                uint64_t bit = e * in.bits;
-                for (; bit < b0 + 64 && e < in.n; bit += in.bits, e++) {
-                    const uint32_t off = (uint32_t)(bit - b0);
-                    uint64_t v = wv[u] >> off;
-                    if (off + in.bits > 64) v |= nv[u] << (64 - off);
-                    acc += v & in.mask;
+                const uint32_t x = r0 + d * 64; // < 2^17
+                const uint32_t rx = x - (uint32_t)(((uint64_t)x * wmagic) >> 32) * w;
+                uint32_t off = rx ? w - rx : 0; // first element start in the word
+                const uint64_t left = nbits - q * 64;
+                const uint32_t lim = left < 64 ? (uint32_t)left : 64; // starts before n*bits
+                const uint32_t l0 = (uint32_t)wv[u], h0 = (uint32_t)(wv[u] >> 32), l1 = (uint32_t)nv[u];
+                for (; off < lim; off += w) {
+                    const uint32_t hi = off < 32 ? h0 : l1, lo = off < 32 ? l0 : h0;
+                    acc += __builtin_amdgcn_alignbit(hi, lo, off & 31) & vm;
                 }
             }
         }
@@ -363,6 +410,12 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
         return check_launch();
     }
     if (c.kind == K_SLAB) c.kind = K_BITSTREAM; // 1-bit slabs are dense streams
+    if (c.kind == K_BITSTREAM && c.bits <= 32) {
+        const uint64_t TS = (c.n + FS_TILE - 1) / FS_TILE;
+        if (TS >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
+        hipLaunchKernelGGL(k_filter_stream, dim3((uint32_t)TS), dim3(256), 0, s, c, pr, mask_out);
+        return check_launch();
+    }
 #define FT_GO(IK, W)                                                                                        \
     {                                                                                                       \
         static uint32_t capb = 0;                                                                           \

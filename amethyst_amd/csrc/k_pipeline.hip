@@ -280,7 +280,7 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
     if (ws_bytes < am_pipeline_workspace_bytes(ns.data(), count)) return AM_E_WORKSPACE;
     std::vector<PipeJob> jobs(count);
     uint64_t tiles = 0, groups = 0;
-    int kind0 = -1;
+    int kind0 = -1; // input format of the batch
     for (uint32_t i = 0; i < count; i++) {
         const am_pipeline_col& c = cols[i];
         PipeJob& J = jobs[i];
@@ -291,8 +291,8 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
         if (J.a.n != J.b.n || c.a.bits != c.b.bits || c.a.format != c.b.format) return AM_E_INVALID;
         if (c.a.bits > 31) return AM_E_UNSUPPORTED_CONFIG;
         if (J.s.kind == K_STRIDED) return AM_E_UNSUPPORTED_CONFIG;
-        if (kind0 < 0) kind0 = (int)J.a.kind;
-        if ((int)J.a.kind != kind0) return AM_E_INVALID; // one input kind per batch
+        if (kind0 < 0) kind0 = (int)c.a.format;
+        if ((int)c.a.format != kind0) return AM_E_INVALID; // one input format per batch (the launch's decoder)
         if (c.pred.op > AM_PRED_NE) return AM_E_INVALID;
         J.pr = make_pred(c.pred, J.a.mask);
         J.stats = c.stats;
