@@ -23,8 +23,8 @@
 
 namespace am {
 
-constexpr uint32_t GB_TILE = 4096;
 constexpr uint32_t GB_THREADS = 1024;
+constexpr uint32_t GB_ROWS = 8; // rows per thread per step (loads in flight)
 
 struct GroupArgs {
     Col keys, vals;
@@ -48,16 +48,17 @@ __device__ inline uint32_t width_of(uint64_t v)
     return v ? 64 - (uint32_t)__clzll(v) : 1;
 }
 
-template <int KK, int VK>
-__global__ __launch_bounds__(GB_THREADS) void k_groupby_partial(GroupArgs a)
+template <int KK, int VK, uint32_t TH>
+__global__ __launch_bounds__(TH) void k_groupby_partial(GroupArgs a)
 {
+    constexpr uint32_t TILE = GB_ROWS * TH; // rows per step
     extern __shared__ uint64_t lds[];
     const uint32_t G = 1u << a.kbits;
     uint64_t* packed = lds;       // [G]
     uint64_t* wsum = lds + G;     // [G]
     uint64_t* wcnt = lds + 2 * G; // [G]
     const uint32_t tid = threadIdx.x;
-    for (uint32_t g = tid; g < G; g += GB_THREADS) {
+    for (uint32_t g = tid; g < G; g += TH) {
         packed[g] = 0;
         wsum[g] = 0;
         wcnt[g] = 0;
@@ -71,27 +72,27 @@ __global__ __launch_bounds__(GB_THREADS) void k_groupby_partial(GroupArgs a)
     const uint64_t SMASK = lowmask(a.SB);
     const uint64_t ONE = 1ull << a.SB;
     uint32_t since = 0;
-    for (uint64_t t0 = r0; t0 < r1; t0 += GB_TILE) {
+    for (uint64_t t0 = r0; t0 < r1; t0 += TILE) {
         TileReader<KK> rk(a.keys, t0);
         TileReader<VK> rv(a.vals, t0);
         const uint64_t rem = r1 - t0;
-        uint64_t kv[GB_TILE / GB_THREADS], vv[GB_TILE / GB_THREADS];
+        uint64_t kv[TILE / TH], vv[TILE / TH];
 #pragma unroll
-        for (uint32_t r = 0; r < GB_TILE / GB_THREADS; r++) {
-            const uint32_t i = r * GB_THREADS + tid;
+        for (uint32_t r = 0; r < TILE / TH; r++) {
+            const uint32_t i = r * TH + tid;
             kv[r] = i < rem ? rk.get(i) : 0;
             vv[r] = i < rem ? rv.get(i) : 0;
         }
 #pragma unroll
-        for (uint32_t r = 0; r < GB_TILE / GB_THREADS; r++) {
-            const uint32_t i = r * GB_THREADS + tid;
+        for (uint32_t r = 0; r < TILE / TH; r++) {
+            const uint32_t i = r * TH + tid;
             if (i < rem && (!a.use_pred || a.pr(vv[r])))
                 atomicAdd((unsigned long long*)&packed[kv[r]], (unsigned long long)(ONE | vv[r]));
         }
         if (++since == a.fold_tiles) {
             since = 0;
             __syncthreads();
-            for (uint32_t g = tid; g < G; g += GB_THREADS) {
+            for (uint32_t g = tid; g < G; g += TH) {
                 const uint64_t p = packed[g];
                 if (p) {
                     wsum[g] += p & SMASK;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(GB_THREADS) void k_groupby_partial(GroupArgs a)
     }
     __syncthreads();
     uint64_t* row = a.part + (uint64_t)blockIdx.x * 2 * G;
-    for (uint32_t g = tid; g < G; g += GB_THREADS) {
+    for (uint32_t g = tid; g < G; g += TH) {
         const uint64_t p = packed[g];
         row[g] = wsum[g] + (p & SMASK);
         row[G + g] = wcnt[g] + (p >> a.SB);
@@ -239,12 +240,20 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     // packed word: sum field SB bits, count field 64-SB bits; R = 2^r rows per fold
     const uint32_t r = (63 - cv.bits) / 2;
     a.SB = cv.bits + r;
-    a.fold_tiles = (uint32_t)((1ull << r) / GB_TILE); // r >= 15 for vbits <= 32, so >= 8 tiles
-    const uint64_t tiles = (ck.n + GB_TILE - 1) / GB_TILE;
-    uint64_t blocks = tiles < GB_MAX_BLOCKS ? (tiles ? tiles : 1) : GB_MAX_BLOCKS;
+    // up to 1024 keys: 4-wave blocks (24 KB of LDS, 6 per CU) and as many as the
+    // partial-table region holds, so enough rows are in flight per CU; wider
+    // keys: 16-wave blocks (LDS-bound), 256 of them
+    const uint32_t G = 1u << ck.bits;
+    const uint32_t TH = GB_THREADS;
+    const uint32_t tile = GB_ROWS * TH;
+    a.fold_tiles = (uint32_t)((1ull << r) / tile); // r >= 15 for vbits <= 32, so >= 8 steps
+    uint64_t maxb = WS_GBPART_BYTES / (16ull * G);
+    if (maxb > 2048) maxb = 2048;
+    const uint64_t tiles = (ck.n + tile - 1) / tile;
+    uint64_t blocks = tiles < maxb ? (tiles ? tiles : 1) : maxb;
     const uint64_t tiles_per_block = tiles ? (tiles + blocks - 1) / blocks : 1;
     blocks = tiles ? (tiles + tiles_per_block - 1) / tiles_per_block : 1;
-    a.rows_per_block = tiles_per_block * GB_TILE;
+    a.rows_per_block = tiles_per_block * tile;
     a.blocks = (uint32_t)blocks;
     a.part = ws_gbpart(ws);
     a.total = ws_small(ws);
@@ -252,14 +261,18 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     a.out_sums = out_sums;
     a.out_counts = out_counts;
     a.result = result_dev;
-    const uint32_t G = 1u << ck.bits;
     const size_t lds = (size_t)G * 24;
     hipStream_t s = (hipStream_t)stream;
-#define GB_LAUNCH(KK, VK)                                                                                      \
+#define GB_GO(KK, VK, T)                                                                                       \
     {                                                                                                          \
-        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_groupby_partial<KK, VK>,                     \
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_groupby_partial<KK, VK, T>,                  \
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);      \
-        hipLaunchKernelGGL((k_groupby_partial<KK, VK>), dim3((uint32_t)blocks), dim3(GB_THREADS), lds, s, a);  \
+        hipLaunchKernelGGL((k_groupby_partial<KK, VK, T>), dim3((uint32_t)blocks), dim3(T), lds, s, a);        \
+    }
+#define GB_LAUNCH(KK, VK)                      \
+    {                                          \
+        if (TH == 256) GB_GO(KK, VK, 256)      \
+        else GB_GO(KK, VK, GB_THREADS)         \
     }
 #define GB_V(KK)                                         \
     switch (cv.kind) {                                   \
@@ -274,6 +287,7 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     }
 #undef GB_V
 #undef GB_LAUNCH
+#undef GB_GO
     hipLaunchKernelGGL(k_groupby_reduce, dim3((2 * G + 63) / 64), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_groupby_writeout, dim3(1), dim3(GB_THREADS), 0, s, a);
     return check_launch();
