@@ -41,6 +41,7 @@ struct GroupArgs {
     uint64_t* out_sums;
     uint64_t* out_counts;
     uint64_t* result;
+    uint32_t vmagic; // word path: floor(2^32 / eps of the values) + 1
 };
 
 __device__ inline uint32_t width_of(uint64_t v)
@@ -106,6 +107,99 @@ __global__ __launch_bounds__(TH) void k_groupby_partial(GroupArgs a)
     __syncthreads();
     uint64_t* row = a.part + (uint64_t)blockIdx.x * 2 * G;
     for (uint32_t g = tid; g < G; g += TH) {
+        const uint64_t p = packed[g];
+        row[g] = wsum[g] + (p & SMASK);
+        row[G + g] = wcnt[g] + (p >> a.SB);
+    }
+}
+
+// Slab keys and slab values, word per lane: per step a thread takes
+// GW_UNITS key words (ek rows each), their key fields at fixed shifts, and
+// the 2..4 value words those rows span, walking the value fields with a
+// small register window (shifted when a word is used up). Per row: two
+// field extracts and the LDS atomic -- no per-row division or load. Same
+// table, folds and partial rows as k_groupby_partial.
+constexpr uint32_t GW_UNITS = 2;
+
+__global__ __launch_bounds__(GB_THREADS) void k_groupby_words(GroupArgs a)
+{
+    extern __shared__ uint64_t lds[];
+    const uint32_t G = 1u << a.kbits;
+    uint64_t* packed = lds;
+    uint64_t* wsum = lds + G;
+    uint64_t* wcnt = lds + 2 * G;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t g = tid; g < G; g += GB_THREADS) {
+        packed[g] = 0;
+        wsum[g] = 0;
+        wcnt[g] = 0;
+    }
+    __syncthreads();
+    const uint64_t n = a.keys.n;
+    const uint32_t ek = a.keys.eps, ev = a.vals.eps, wk = a.keys.bits, wv = a.vals.bits;
+    const uint64_t km = a.keys.mask, vm = a.vals.mask;
+    const uint64_t vwords = a.vals.words, kwords = a.keys.words;
+    const uint64_t r0 = (uint64_t)blockIdx.x * a.rows_per_block; // a multiple of ek
+    uint64_t r1 = r0 + a.rows_per_block;
+    if (r1 > n) r1 = n;
+    const uint64_t SMASK = lowmask(a.SB);
+    const uint64_t ONE = 1ull << a.SB;
+    const uint32_t step = GB_THREADS * GW_UNITS * ek; // rows per step (<= one fold period)
+    uint32_t since = 0;
+    for (uint64_t t0 = r0; t0 < r1; t0 += step) {
+        const uint64_t qv0 = t0 / ev; // value word and field of row t0 (uniform)
+        const uint32_t sv0 = (uint32_t)(t0 - qv0 * ev);
+        uint64_t K[GW_UNITS], V[GW_UNITS][4];
+        uint32_t S[GW_UNITS];
+#pragma unroll
+        for (uint32_t u = 0; u < GW_UNITS; u++) {
+            const uint32_t rel = (u * GB_THREADS + tid) * ek; // < 2^24
+            const uint64_t kq = t0 / ek + u * GB_THREADS + tid;
+            K[u] = kq < kwords ? a.keys.p[kq] : 0;
+            const uint32_t x = sv0 + rel;
+            const uint32_t dq = ev == 1 ? x : (uint32_t)(((uint64_t)x * a.vmagic) >> 32);
+            S[u] = x - dq * ev;
+            const uint64_t qv = qv0 + dq;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) V[u][k] = qv + k < vwords ? a.vals.p[qv + k] : 0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < GW_UNITS; u++) {
+            const uint64_t e0 = t0 + (u * GB_THREADS + tid) * ek;
+            if (e0 >= r1) continue;
+            const uint32_t cnt = (uint32_t)(r1 - e0 < ek ? r1 - e0 : ek);
+            uint64_t v0 = V[u][0], v1 = V[u][1], v2 = V[u][2], v3 = V[u][3];
+            uint32_t sv = S[u];
+            for (uint32_t j = 0, sk = 0; j < cnt; j++, sk += wk) {
+                const uint64_t key = (K[u] >> sk) & km;
+                const uint64_t val = (v0 >> (sv * wv)) & vm;
+                if (!a.use_pred || a.pr(val))
+                    atomicAdd((unsigned long long*)&packed[key], (unsigned long long)(ONE | val));
+                if (++sv == ev) {
+                    sv = 0;
+                    v0 = v1;
+                    v1 = v2;
+                    v2 = v3;
+                }
+            }
+        }
+        if (++since == a.fold_tiles) {
+            since = 0;
+            __syncthreads();
+            for (uint32_t g = tid; g < G; g += GB_THREADS) {
+                const uint64_t p = packed[g];
+                if (p) {
+                    wsum[g] += p & SMASK;
+                    wcnt[g] += p >> a.SB;
+                    packed[g] = 0;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    uint64_t* row = a.part + (uint64_t)blockIdx.x * 2 * G;
+    for (uint32_t g = tid; g < G; g += GB_THREADS) {
         const uint64_t p = packed[g];
         row[g] = wsum[g] + (p & SMASK);
         row[G + g] = wcnt[g] + (p >> a.SB);
@@ -263,6 +357,25 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     a.result = result_dev;
     const size_t lds = (size_t)G * 24;
     hipStream_t s = (hipStream_t)stream;
+    // word path: slab keys and values, a key word's rows within 4 value words,
+    // a step's rows within one fold period (the packed count cannot overflow)
+    const uint32_t wstep = GB_THREADS * GW_UNITS * ck.eps;
+    if (ck.kind == K_SLAB && cv.kind == K_SLAB && ck.eps + cv.eps - 1 <= 4 * cv.eps && wstep <= (1u << r) &&
+        ck.n) {
+        a.fold_tiles = (uint32_t)((1ull << r) / wstep);
+        a.vmagic = (uint32_t)((1ull << 32) / cv.eps + 1);
+        uint64_t steps = (ck.n + wstep - 1) / wstep;
+        uint64_t wb = steps < maxb ? steps : maxb;
+        const uint64_t spb = (steps + wb - 1) / wb;
+        wb = (steps + spb - 1) / spb;
+        a.rows_per_block = spb * wstep; // a multiple of the key word's rows
+        a.blocks = (uint32_t)wb;
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_groupby_words, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_groupby_words, dim3((uint32_t)wb), dim3(GB_THREADS), lds, s, a);
+        hipLaunchKernelGGL(k_groupby_reduce, dim3((2 * G + 63) / 64), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_groupby_writeout, dim3(1), dim3(GB_THREADS), 0, s, a);
+        return check_launch();
+    }
 #define GB_GO(KK, VK, T)                                                                                       \
     {                                                                                                          \
         if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_groupby_partial<KK, VK, T>,                  \

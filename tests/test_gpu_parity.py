@@ -354,6 +354,30 @@ def _pack_matrix(oracle_mod, X, bits):
     return np.concatenate([oracle_mod.pack(1, bits, X[r]) for r in range(X.shape[0])])
 
 
+def test_groupby_folds(am, oracle_mod):
+    """Enough rows that every block of the group-by partials folds its packed
+    LDS table (12-bit keys, 32-bit values: a fold every 32768 rows)."""
+    rng = np.random.default_rng(19)
+    ws = am.Workspace(0)
+    kbits, vbits, n = 12, 32, 9000001
+    keys = rng.integers(0, 1 << kbits, size=n, dtype=np.uint64)
+    vals = rand_vals(rng, n, vbits)
+    kc = upload(am, oracle_mod, keys, kbits, 1)
+    vc = upload(am, oracle_mod, vals, vbits, 1)
+    say("groupby-folds", kbits, vbits, n)
+    ok, osum, ocnt, res = am.groupby(kc, vc, mode=0, ws=ws)
+    r = am.u64(res)
+    sums = np.zeros(1 << kbits, np.uint64)
+    np.add.at(sums, keys.astype(np.int64), vals)
+    counts = np.bincount(keys.astype(np.int64), minlength=1 << kbits).astype(np.uint64)
+    present = np.nonzero(counts)[0]
+    ng = len(present)
+    assert int(r[0]) == ng and int(r[3]) == n
+    assert np.array_equal(oracle_mod.unpack(1, kbits, am.u64(ok), ng), present.astype(np.uint64))
+    assert np.array_equal(oracle_mod.unpack(1, int(r[1]), am.u64(osum), ng), sums[present])
+    assert np.array_equal(oracle_mod.unpack(1, int(r[2]), am.u64(ocnt), ng), counts[present])
+
+
 def test_matmul_packed(am, oracle_mod):
     import torch
     rng = np.random.default_rng(10)
