@@ -454,8 +454,14 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
                 const uint32_t wa = (lane * 64) / eps;       // the word holding the mask word's first element
                 const uint32_t sk = lane * 64 - wa * eps;    // its elements before that one
                 uint64_t m = sg[wa] >> sk;
-                for (uint32_t w = wa + 1, sh = eps - sk; sh < 64 && w < 64 * R; w++, sh += eps)
-                    m |= (uint64_t)sg[w] << sh;
+                uint32_t w = wa + 1, sh = eps - sk;
+                // two words per step (independent loads: half the LDS round
+                // trips); both start inside the mask word, so both exist
+                for (; sh + eps < 64; w += 2, sh += 2 * eps) {
+                    const uint32_t g0 = sg[w], g1 = sg[w + 1];
+                    m |= ((uint64_t)g0 << sh) | ((uint64_t)g1 << (sh + eps));
+                }
+                if (sh < 64) m |= (uint64_t)sg[w] << sh;
                 const uint64_t mi = mw0 + (uint64_t)wave * nwm + lane;
                 if (mi < nmw) a.mask_out[mi] = m; // bits past n are zero (cleared above)
             }
