@@ -9,11 +9,11 @@
 //
 // Pipeline (stream-ordered):
 //   1. unpack A (M x K) to int8 rows [M][Kp] and B (K x N) to int8 columns
-//      [N][Kp] (Kp = K rounded up to 64, zero-padded): 7-bit values are
+//      [N][Kp] (Kp = K rounded up to 128, zero-padded): 7-bit values are
 //      non-negative in int8, so the signed int8 MFMA is exact;
-//   2. GEMM: 128x128 block tiles, K-steps of 64 staged through LDS with
-//      16-byte loads and double buffering; each of 4 waves owns a 64x64
-//      quadrant = 4x4 MFMA tiles of 16x16 (i32 accumulators).
+//   2. GEMM: 256x256 block tiles, K-steps of 128 staged through LDS with
+//      16-byte loads and double buffering; each of 4 waves owns a 128x128
+//      quadrant = 8x8 MFMA tiles of 16x16 (i32 accumulators);
 //      Both operands use the same k-to-slot mapping per lane, so the sum over
 //      k is exact whatever order the matrix core visits the slots in;
 //   3. epilogue: C as row-major int32 and/or re-packed as a row-padded slab
@@ -31,8 +31,12 @@ namespace am {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t MM_BM = 128, MM_BN = 128, MM_BK = 64;
-constexpr uint32_t MM_LDS_STRIDE = MM_BK + 16; // bytes per staged row (bank spread)
+constexpr uint32_t MM_BM = 256, MM_BN = 256, MM_BK = 128, MM_GROUP = 8;
+constexpr int MM_RSRC_FLAGS = 0x00020000;        // buffer descriptor word 3: 32-bit data format
+constexpr uint32_t MM_STAGE = (MM_BM + MM_BN) * MM_BK; // bytes of one K-step stage (unpadded rows)
+constexpr uint32_t MM_C_STRIDE = 260;            // int32 per staged C row (4*stride = 16 mod 32 banks)
+constexpr uint32_t MM_LDS = 128 * MM_C_STRIDE * 4 > 2 * MM_STAGE ? 128 * MM_C_STRIDE * 4 : 2 * MM_STAGE;
+constexpr unsigned MM_WAIT_VM0 = 0x0F70;         // s_waitcnt vmcnt(0), other counters unconstrained
 
 struct MatJob {
     const uint64_t* A; // packed M x K
@@ -43,7 +47,7 @@ struct MatJob {
     int8_t* B8;        // [N][Kp]
     uint32_t M, N, K, Kp;
     uint32_t out_bits;
-    uint32_t tiles_n;  // ceil(N / 128)
+    uint32_t tiles_n;  // ceil(N / 256)
     uint32_t tile0;    // first GEMM tile of this job in the batch
     uint32_t achunk0;  // first A-unpack work item
     uint32_t bchunk0;  // first B-unpack work item
@@ -120,99 +124,187 @@ __global__ __launch_bounds__(256) void k_mm_unpack_bt(const MatJob* jobs, uint32
     }
 }
 
-// GEMM over the batch's tile list; one 128x128 output tile per iteration.
-__global__ __launch_bounds__(256) void k_mm_gemm(const MatJob* jobs, uint32_t njobs, uint32_t total_tiles)
+// Tile id -> (job, m0, n0). Tiles of one job are taken in groups of
+// MM_GROUP tile rows (column-major inside a group), and the persistent
+// grid's blocks are renumbered so that the blocks of one XCD (block b runs
+// on XCD b % 8) take consecutive tiles: at any time one XCD works on a few
+// A row-tiles and B column-tiles, whose K-steps its L2 then serves once.
+__device__ inline uint32_t mm_logical_tile(uint32_t tile, uint32_t grid, uint32_t total)
 {
-    __shared__ __attribute__((aligned(16))) int8_t sA[2][MM_BM * MM_LDS_STRIDE];
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][MM_BN * MM_LDS_STRIDE];
+    // a permutation inside each full round of `grid` tiles; a partial last
+    // round keeps the identity so every id stays below `total`
+    const uint32_t round = tile / grid, b = tile % grid;
+    if (grid % 8 != 0 || (uint64_t)(round + 1) * grid > total) return tile;
+    return round * grid + (b % 8) * (grid / 8) + b / 8;
+}
+
+__device__ inline uint32_t rfl(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+template <typename T> __device__ inline T* rfl_ptr(T* p)
+{
+    const uint64_t v = (uint64_t)p;
+    return (T*)(((uint64_t)rfl((uint32_t)(v >> 32)) << 32) | rfl((uint32_t)v));
+}
+
+// The job's fields as wave-uniform (scalar) values: the job list is read with
+// vector loads, and buffer descriptors must be built from scalars.
+__device__ inline MatJob uniform_job(const MatJob& j)
+{
+    MatJob u;
+    u.A = rfl_ptr(j.A);
+    u.B = rfl_ptr(j.B);
+    u.Cp = rfl_ptr(j.Cp);
+    u.Ci = rfl_ptr(j.Ci);
+    u.A8 = rfl_ptr(j.A8);
+    u.B8 = rfl_ptr(j.B8);
+    u.M = rfl(j.M);
+    u.N = rfl(j.N);
+    u.K = rfl(j.K);
+    u.Kp = rfl(j.Kp);
+    u.out_bits = rfl(j.out_bits);
+    u.tiles_n = rfl(j.tiles_n);
+    u.tile0 = rfl(j.tile0);
+    u.achunk0 = rfl(j.achunk0);
+    u.bchunk0 = rfl(j.bchunk0);
+    return u;
+}
+
+// GEMM over the batch's tile list; one 256x256 output tile per iteration.
+// 4 waves in a 2x2 arrangement, each owning 128x128 = 8x8 MFMA tiles of
+// 16x16 (256 i32 accumulators per lane). K-steps of 128 bytes are staged in
+// LDS (double-buffered, 144 KB, one block per CU); the next step's global
+// loads are in flight in registers while the current step computes.
+__global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t njobs, uint32_t total_tiles)
+{
+    // two K-step stages [A rows | B rows]; the epilogue reuses it as 128 int32 rows
+    __shared__ __attribute__((aligned(1024))) int8_t smem[MM_LDS];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wm = wave >> 1, wn = wave & 1;
     const uint32_t lr = lane & 15, lg = lane >> 4;
+    auto sA = [&](uint32_t buf) { return smem + buf * MM_STAGE; };
+    auto sB = [&](uint32_t buf) { return smem + buf * MM_STAGE + MM_BM * MM_BK; };
 
     for (uint32_t tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
-        const MatJob& J = jobs[find_job(jobs, njobs, tile, 2)];
-        const uint32_t local = tile - J.tile0;
-        const uint32_t m0 = (local / J.tiles_n) * MM_BM, n0 = (local % J.tiles_n) * MM_BN;
+        const uint32_t lt = mm_logical_tile(tile, gridDim.x, total_tiles);
+        const MatJob J = uniform_job(jobs[find_job(jobs, njobs, lt, 2)]);
+        const uint32_t local = lt - J.tile0;
+        const uint32_t tiles_m = (J.M + MM_BM - 1) / MM_BM;
+        const uint32_t per_group = MM_GROUP * J.tiles_n, grp = local / per_group;
+        const uint32_t first = grp * MM_GROUP, gsz = min(tiles_m - first, MM_GROUP), r = local % per_group;
+        const uint32_t m0 = (first + r % gsz) * MM_BM, n0 = (r / gsz) * MM_BN;
         const uint32_t steps = J.Kp / MM_BK;
 
-        // each thread stages 2 x 16 bytes of A and of B per K-step
-        uint4 ra[2], rb[2];
-        auto gload = [&](uint32_t ks) {
+        // K-steps land in LDS straight from global memory (buffer loads with
+        // the LDS flag, 16 bytes per lane, no registers): each instruction
+        // fills 1 KB = 8 rows of 128 bytes, lane l the 16-byte slot l%8 of row
+        // l/8, and slot s of row R holds the row's chunk s ^ (R%8) (an XOR
+        // swizzle: the fragment reads of 8 consecutive rows then hit 8
+        // different slots). Wave w loads rows 64w..64w+63 of A and of B.
+        // Loads bounded by the tile's valid rows read zeros past M (or N).
+        const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(J.A8 + (uint64_t)m0 * J.Kp), (short)0, (int)(min(J.M - m0, MM_BM) * J.Kp), MM_RSRC_FLAGS);
+        const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(J.B8 + (uint64_t)n0 * J.Kp), (short)0, (int)(min(J.N - n0, MM_BN) * J.Kp), MM_RSRC_FLAGS);
+        const uint32_t lrow = lane >> 3;
+        const uint32_t vo = (wave * 64 + lrow) * J.Kp + ((lane & 7) ^ lrow) * 16, rs = 8 * J.Kp;
+        auto gload = [&](uint32_t ks, uint32_t buf) {
+            const uint32_t ko = vo + ks * MM_BK;
 #pragma unroll
-            for (uint32_t h = 0; h < 2; h++) {
-                const uint32_t c = tid + h * 256; // 512 chunks of 16 B per operand
-                const uint32_t row = c >> 2, kq = (c & 3) * 16;
-                const uint32_t gm = m0 + row, gn = n0 + row;
-                ra[h] = gm < J.M ? *(const uint4*)(J.A8 + (uint64_t)gm * J.Kp + ks * MM_BK + kq) : make_uint4(0, 0, 0, 0);
-                rb[h] = gn < J.N ? *(const uint4*)(J.B8 + (uint64_t)gn * J.Kp + ks * MM_BK + kq) : make_uint4(0, 0, 0, 0);
+            for (uint32_t h = 0; h < 8; h++) {
+                const uint32_t lo = (wave * 8 + h) * 1024;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (__attribute__((address_space(3))) void*)(sA(buf) + lo),
+                                                         16, ko + h * rs, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (__attribute__((address_space(3))) void*)(sB(buf) + lo),
+                                                         16, ko + h * rs, 0, 0, 0);
             }
         };
-        auto sstore = [&](uint32_t buf) {
-#pragma unroll
-            for (uint32_t h = 0; h < 2; h++) {
-                const uint32_t c = tid + h * 256;
-                const uint32_t row = c >> 2, kq = (c & 3) * 16;
-                *(uint4*)(&sA[buf][row * MM_LDS_STRIDE + kq]) = ra[h];
-                *(uint4*)(&sB[buf][row * MM_LDS_STRIDE + kq]) = rb[h];
-            }
-        };
+        // fragment reads: row R = 128*wm (or wn) + 16*i + lr, chunk 4*sub + lg
+        const uint32_t fo0 = lr * MM_BK + ((lg ^ (lr & 7)) * 16), fo1 = lr * MM_BK + (((4 + lg) ^ (lr & 7)) * 16);
 
-        v4i acc[4][4];
+        v4i acc[8][8];
 #pragma unroll
-        for (uint32_t i = 0; i < 4; i++)
+        for (uint32_t i = 0; i < 8; i++)
 #pragma unroll
-            for (uint32_t j = 0; j < 4; j++) acc[i][j] = v4i{0, 0, 0, 0};
+            for (uint32_t j = 0; j < 8; j++) acc[i][j] = v4i{0, 0, 0, 0};
 
-        gload(0);
-        __syncthreads(); // previous tile's readers are done with the buffers
-        sstore(0);
+        __syncthreads(); // previous tile's epilogue is done with the LDS
+        gload(0, 0);
+        __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
         __syncthreads();
         for (uint32_t ks = 0; ks < steps; ks++) {
             const uint32_t buf = ks & 1;
-            if (ks + 1 < steps) gload(ks + 1);
-            v4i fa[4], fb[4];
+            if (ks + 1 < steps) gload(ks + 1, buf ^ 1); // read in step ks-1, released by its barrier
 #pragma unroll
-            for (uint32_t i = 0; i < 4; i++) {
-                fa[i] = *(const v4i*)(&sA[buf][(wm * 64 + i * 16 + lr) * MM_LDS_STRIDE + lg * 16]);
-                fb[i] = *(const v4i*)(&sB[buf][(wn * 64 + i * 16 + lr) * MM_LDS_STRIDE + lg * 16]);
+            for (uint32_t sub = 0; sub < MM_BK / 64; sub++) {
+                const uint32_t fo = sub ? fo1 : fo0;
+                v4i fa[8], fb[8];
+#pragma unroll
+                for (uint32_t i = 0; i < 8; i++) {
+                    fa[i] = *(const v4i*)(sA(buf) + (wm * 128 + i * 16) * MM_BK + fo);
+                    fb[i] = *(const v4i*)(sB(buf) + (wn * 128 + i * 16) * MM_BK + fo);
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < 8; i++)
+#pragma unroll
+                    for (uint32_t j = 0; j < 8; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
             }
-#pragma unroll
-            for (uint32_t i = 0; i < 4; i++)
-#pragma unroll
-                for (uint32_t j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-            if (ks + 1 < steps) {
-                sstore(buf ^ 1);
-                __syncthreads();
-            }
+            __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0); // this wave's loads of the next step have landed
+            __syncthreads();                         // ... and every wave's, and step ks is read
         }
 
-        // ---- epilogue: lane holds C[row0 + 4*lg + r][col0 + lr], r < 4
+        // ---- epilogue through LDS, one 128-row half of the tile at a time:
+        // the two waves owning the half write their accumulators (lane holds
+        // C[row0 + 4*lg + r][col0 + lr], r < 4) as int32 rows, then all
+        // threads store whole rows: int32 C in 16-byte pieces and/or packed
+        // words (when floor(64/out_bits) divides 16, words never straddle
+        // tiles; other widths are packed later from the int32 copy)
         const uint32_t ob = J.out_bits;
         const uint32_t eps_o = 64 / ob;
         const bool fused = J.Cp && (16 % eps_o == 0);
         const uint32_t rw_o = (J.N + eps_o - 1) / eps_o;
+        const uint64_t omask = lowmask(ob);
+        int32_t* sc = (int32_t*)smem;
+        __syncthreads(); // every wave is done reading the last K-step
 #pragma unroll
-        for (uint32_t i = 0; i < 4; i++) {
+        for (uint32_t half = 0; half < 2; half++) {
+            if (wm == half) {
 #pragma unroll
-            for (uint32_t j = 0; j < 4; j++) {
-                const uint32_t col = n0 + wn * 64 + j * 16 + lr;
+                for (uint32_t i = 0; i < 8; i++)
 #pragma unroll
-                for (uint32_t r = 0; r < 4; r++) {
-                    const uint32_t row = m0 + wm * 64 + i * 16 + 4 * lg + r;
-                    const int32_t val = acc[i][j][r];
-                    if (J.Ci && row < J.M && col < J.N) J.Ci[(uint64_t)row * J.N + col] = val;
-                    if (fused) {
-                        uint64_t word = (col < J.N) ? ((uint64_t)(uint32_t)val & lowmask(ob)) : 0;
-                        for (uint32_t s = 1; s < eps_o; s++) {
-                            const uint32_t src = __shfl((int)val, (int)(lane + s), 64);
-                            if (col + s < J.N) word |= ((uint64_t)src & lowmask(ob)) << (s * ob);
-                        }
-                        if (row < J.M && col < J.N && (lr % eps_o) == 0)
-                            J.Cp[(uint64_t)row * rw_o + col / eps_o] = word;
+                    for (uint32_t j = 0; j < 8; j++)
+#pragma unroll
+                        for (uint32_t r = 0; r < 4; r++)
+                            sc[(i * 16 + 4 * lg + r) * MM_C_STRIDE + wn * 128 + j * 16 + lr] = acc[i][j][r];
+            }
+            __syncthreads();
+            const uint32_t rbase = m0 + half * 128;
+            if (J.Ci) {
+                for (uint32_t t = tid; t < 128 * 64; t += 256) {
+                    const uint32_t row = t >> 6, c = n0 + (t & 63) * 4, gr = rbase + row;
+                    if (gr >= J.M || c >= J.N) continue;
+                    const int32_t* src = sc + row * MM_C_STRIDE + (t & 63) * 4;
+                    int32_t* dst = J.Ci + (uint64_t)gr * J.N + c;
+                    if (c + 4 <= J.N && ((uintptr_t)dst & 15) == 0) {
+                        *(v4i*)dst = *(const v4i*)src;
+                    } else {
+                        for (uint32_t q = 0; q < 4 && c + q < J.N; q++) dst[q] = src[q];
                     }
                 }
             }
+            if (fused) {
+                const uint32_t lw = 31 - __builtin_clz(256 / eps_o); // log2 words per tile row
+                for (uint32_t t = tid; t < (128u << lw); t += 256) {
+                    const uint32_t row = t >> lw, w = t & ((1u << lw) - 1), gr = rbase + row;
+                    const uint32_t c0 = n0 + w * eps_o;
+                    if (gr >= J.M || c0 >= J.N) continue;
+                    const int32_t* src = sc + row * MM_C_STRIDE + w * eps_o;
+                    uint64_t word = 0;
+                    for (uint32_t q = 0; q < eps_o && c0 + q < J.N; q++)
+                        word |= ((uint64_t)(uint32_t)src[q] & omask) << (q * ob);
+                    J.Cp[(uint64_t)gr * rw_o + c0 / eps_o] = word;
+                }
+            }
+            __syncthreads(); // the half's rows are stored before the next half lands
         }
     }
 }
@@ -257,7 +349,7 @@ using namespace am;
 // Workspace bytes for one matrix product (int8 copies, int32 staging, job list).
 static uint64_t mm_ws_one(uint64_t M, uint64_t N, uint64_t K)
 {
-    const uint64_t Kp = (K + 63) / 64 * 64;
+    const uint64_t Kp = (K + MM_BK - 1) / MM_BK * MM_BK;
     return (M * Kp + 255) / 256 * 256 + (N * Kp + 255) / 256 * 256 + (M * N * 4 + 255) / 256 * 256;
 }
 
@@ -306,7 +398,7 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
         J.M = d.M;
         J.N = d.N;
         J.K = d.K;
-        J.Kp = (d.K + 63) / 64 * 64;
+        J.Kp = (d.K + MM_BK - 1) / MM_BK * MM_BK;
         J.out_bits = d.out_bits ? d.out_bits : am_matmul_out_bits(bits, d.K);
         J.A8 = (int8_t*)cur;
         cur += ((uint64_t)J.M * J.Kp + 255) / 256 * 256;

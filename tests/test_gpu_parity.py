@@ -402,6 +402,32 @@ def test_matmul_packed(am, oracle_mod):
             assert np.array_equal(am.u64(cp)[:len(exp_words)], exp_words), (bits, M, N, K, ob)
 
 
+def test_matmul_single_large(am, oracle_mod):
+    """One matmul with M, N, K >= 512 (the library int8 GEMM + row packer path),
+    with and without the int32 copy; ragged N and K."""
+    import torch
+    rng = np.random.default_rng(13)
+    for bits, (M, N, K), want in ((7, (520, 777, 600), True), (5, (1024, 513, 2000), False),
+                                  (1, (512, 640, 4100), True)):
+        A = rng.integers(0, 1 << bits, (M, K)).astype(np.uint64)
+        B = rng.integers(0, 1 << bits, (K, N)).astype(np.uint64)
+        pa, pb = _pack_matrix(oracle_mod, A, bits), _pack_matrix(oracle_mod, B, bits)
+        # entries < 2^31 are exact in float64 products summed in float64
+        C = np.rint(A.astype(np.float64) @ B.astype(np.float64)).astype(np.int64)
+        if M * N * K <= 520 * 777 * 600:
+            assert np.array_equal(oracle_mod.matmul_packed(bits, pa, pb, M, N, K), C)
+        say("matmul single", bits, M, N, K, want)
+        cp, ci, ob = am.matmul_packed_batched(
+            bits, [(torch.from_numpy(pa.view(np.int64)).cuda(), torch.from_numpy(pb.view(np.int64)).cuda(), M, N, K)],
+            want_i32=want)[0]
+        torch.cuda.synchronize()
+        if want:
+            assert np.array_equal(ci.cpu().numpy().reshape(M, N).astype(np.int64), C), (bits, M, N, K)
+        assert ob == _width(K * ((1 << bits) - 1) ** 2)
+        exp_words = _pack_matrix(oracle_mod, C.astype(np.uint64), ob)
+        assert np.array_equal(am.u64(cp)[:len(exp_words)], exp_words), (bits, M, N, K, ob)
+
+
 def test_matmul_uneven_batch_packed_only(am, oracle_mod):
     """A batch of uneven shapes with packed output only (row packer path)."""
     import torch
