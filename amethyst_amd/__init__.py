@@ -398,23 +398,35 @@ def matrix_words(rows, cols, bits):
 def matmul_packed_batched(bits, problems, want_i32=False, out_bits=None, stream=None):
     """problems: list of (A_words_tensor, B_words_tensor, M, N, K).
 
-    Returns a list of (C_packed tensor, C_i32 tensor or None, out_bits)."""
+    Returns a list of (C_packed tensor, C_i32 tensor or None, out_bits).
+    Every word of every output is written by the library, so the outputs and
+    the workspace are left uninitialised (one allocation each)."""
     torch = _torch()
     n = len(problems)
     Ms = (ctypes.c_uint32 * n)(*[p[2] for p in problems])
     Ns = (ctypes.c_uint32 * n)(*[p[3] for p in problems])
     Ks = (ctypes.c_uint32 * n)(*[p[4] for p in problems])
     wsb = int(lib().am_matmul_workspace_bytes(Ms, Ns, Ks, n))
-    ws = device_words((wsb + 7) // 8)
+    ws = torch.empty((wsb + 7) // 8, dtype=torch.int64, device="cuda")
+    obs = [out_bits if out_bits else matmul_out_bits(bits, p[4]) for p in problems]
+    pw = [matrix_words(M, N, ob) for (_, _, M, N, _), ob in zip(problems, obs)]
+    tot_p = tot_c = 0  # (the module's own sum() shadows the builtin)
+    for (_, _, M, N, _), w in zip(problems, pw):
+        tot_p += w
+        tot_c += M * N
+    cp_all = torch.empty(max(1, tot_p), dtype=torch.int64, device="cuda")
+    ci_all = torch.empty(max(1, tot_c), dtype=torch.int32, device="cuda") if want_i32 else None
     descs = (am_matmul_desc * n)()
     outs = []
+    po = co = 0
     for i, (A, B, M, N, K) in enumerate(problems):
-        ob = out_bits if out_bits else matmul_out_bits(bits, K)
-        cp = device_words(matrix_words(M, N, ob))
-        ci = torch.zeros(M * N, dtype=torch.int32, device="cuda") if want_i32 else None
+        cp = cp_all[po:po + pw[i]]
+        ci = ci_all[co:co + M * N] if want_i32 else None
+        po += pw[i]
+        co += M * N
         descs[i] = am_matmul_desc(A.data_ptr(), B.data_ptr(), cp.data_ptr(), ci.data_ptr() if ci is not None else None,
-                                  M, N, K, ob)
-        outs.append((cp, ci, ob))
+                                  M, N, K, obs[i])
+        outs.append((cp, ci, obs[i]))
     _check(lib().am_matmul_packed_batched(bits, descs, n, ctypes.c_void_p(ws.data_ptr()), wsb, _stream(stream)),
            "am_matmul_packed_batched")
     outs.append(ws)  # keep the workspace alive until the caller synchronises

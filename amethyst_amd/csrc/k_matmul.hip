@@ -11,9 +11,9 @@
 //   1. unpack A (M x K) to int8 rows [M][Kp] and B (K x N) to int8 columns
 //      [N][Kp] (Kp = K rounded up to 128, zero-padded): 7-bit values are
 //      non-negative in int8, so the signed int8 MFMA is exact;
-//   2. GEMM: 256x256 block tiles, K-steps of 128 staged through LDS with
-//      16-byte loads and double buffering; each of 4 waves owns a 128x128
-//      quadrant = 8x8 MFMA tiles of 16x16 (i32 accumulators);
+//   2. GEMM: 256x256 block tiles, K-steps of 64 loaded straight into a
+//      4-stage LDS ring (3 steps in flight); each of 4 waves owns a 128x128
+//      quadrant = 8x8 MFMA tiles of 16x16 (i32 accumulators).
 //      Both operands use the same k-to-slot mapping per lane, so the sum over
 //      k is exact whatever order the matrix core visits the slots in;
 //   3. epilogue: C as row-major int32 and/or re-packed as a row-padded slab
@@ -22,6 +22,8 @@
 // The batched entry point runs every phase for all matrices of a batch in
 // one persistent launch per phase (work lists built on the host).
 #include <algorithm>
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "am_device.hpp"
@@ -31,12 +33,16 @@ namespace am {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t MM_BM = 256, MM_BN = 256, MM_BK = 128, MM_GROUP = 8;
+constexpr uint32_t MM_BM = 256, MM_BN = 256, MM_BK = 64, MM_GROUP = 8, MM_NSTAGE = 4;
+constexpr uint32_t MM_KPAD = 128; // K padding of the int8 copies (whole B-unpack tiles)
 constexpr int MM_RSRC_FLAGS = 0x00020000;        // buffer descriptor word 3: 32-bit data format
 constexpr uint32_t MM_STAGE = (MM_BM + MM_BN) * MM_BK; // bytes of one K-step stage (unpadded rows)
 constexpr uint32_t MM_C_STRIDE = 260;            // int32 per staged C row (4*stride = 16 mod 32 banks)
-constexpr uint32_t MM_LDS = 128 * MM_C_STRIDE * 4 > 2 * MM_STAGE ? 128 * MM_C_STRIDE * 4 : 2 * MM_STAGE;
-constexpr unsigned MM_WAIT_VM0 = 0x0F70;         // s_waitcnt vmcnt(0), other counters unconstrained
+constexpr uint32_t MM_SLOT = 32 * MM_C_STRIDE * 4; // one stage; in the epilogue 32 rows of C
+static_assert(MM_SLOT >= MM_STAGE, "a stage fits its slot");
+// s_waitcnt vmcnt(n) with the other counters unconstrained (vmcnt bits 3:0 and 15:14)
+constexpr unsigned MM_WAIT_VM0 = 0x0F70, MM_WAIT_VM8 = 0x0F78, MM_WAIT_VM16 = 0x4F70;
+static_assert(MM_NSTAGE == 4, "the vmcnt waits assume 3 steps in flight, 8 loads per step and wave");
 
 struct MatJob {
     const uint64_t* A; // packed M x K
@@ -49,78 +55,111 @@ struct MatJob {
     uint32_t out_bits;
     uint32_t tiles_n;  // ceil(N / 256)
     uint32_t tile0;    // first GEMM tile of this job in the batch
-    uint32_t achunk0;  // first A-unpack work item
-    uint32_t bchunk0;  // first B-unpack work item
 };
 
-__device__ inline uint32_t find_job(const MatJob* jobs, uint32_t njobs, uint32_t item, int which)
+// One block of an unpack launch: its job and its first work unit there
+// (host-built table, so blocks never search the job list).
+struct MatSeg {
+    uint32_t job, first;
+};
+
+constexpr uint32_t MU_A_ITEMS = 1024; // A-unpack items (one row, 16 k) per block
+constexpr uint32_t MU_KT = 128;       // B-unpack tile: k rows
+constexpr uint32_t MU_NT = 576;       // B-unpack tile: at most this many columns (whole words)
+
+__device__ inline uint32_t find_job(const MatJob* jobs, uint32_t njobs, uint32_t tile)
 {
-    // last job whose first item <= item (jobs sorted by first item)
+    // last job whose first tile <= tile (jobs sorted by first tile)
     uint32_t lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        const uint32_t f = which == 0 ? jobs[mid].achunk0 : (which == 1 ? jobs[mid].bchunk0 : jobs[mid].tile0);
-        if (f <= item) lo = mid;
+        if (jobs[mid].tile0 <= tile) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
 
-__device__ inline uint32_t get7(const uint64_t* row, uint32_t c, uint32_t bits, uint32_t eps)
+// A -> int8 rows [M][Kp]: one item = one row and 16 consecutive k (one
+// 16-byte store). The 16 fields span at most 3 packed words (>= 9 fields per
+// word); neighbouring lanes take neighbouring items of a row, so the word
+// loads of a wave cover a contiguous stretch of the row.
+template <uint32_t BITS>
+__global__ __launch_bounds__(256) void k_mm_unpack_a(const MatJob* jobs, const MatSeg* segs)
 {
-    const uint64_t w = row[c / eps];
-    return (uint32_t)((w >> ((c % eps) * bits)) & lowmask(bits));
-}
-
-// A: one work item = one row and 16 consecutive k (one 16-byte store).
-__global__ __launch_bounds__(256) void k_mm_unpack_a(const MatJob* jobs, uint32_t njobs, uint32_t bits,
-                                                     uint32_t total_items)
-{
-    const uint32_t eps = 64 / bits;
-    for (uint32_t it = blockIdx.x * 256 + threadIdx.x; it < total_items; it += gridDim.x * 256) {
-        const MatJob& J = jobs[find_job(jobs, njobs, it, 0)];
-        const uint32_t local = it - J.achunk0;
-        const uint32_t cpr = J.Kp / 16; // chunks per row
-        const uint32_t r = local / cpr, k0 = (local % cpr) * 16;
-        const uint64_t* row = J.A + (uint64_t)r * ((J.K + eps - 1) / eps);
+    constexpr uint32_t EPS = 64 / BITS;
+    constexpr uint64_t MASK = (1ull << BITS) - 1;
+    const MatSeg sg = segs[blockIdx.x];
+    const MatJob& J = jobs[sg.job];
+    const uint32_t cpr = J.Kp / 16, rw = (J.K + EPS - 1) / EPS;
+    const uint32_t items = J.M * cpr; // < 2^32 (host-checked)
+#pragma unroll
+    for (uint32_t q = 0; q < MU_A_ITEMS / 256; q++) {
+        const uint32_t it = sg.first + q * 256 + threadIdx.x;
+        if (it >= items) break;
+        const uint32_t r = it / cpr, k0 = (it - r * cpr) * 16;
         uint32_t v[4] = {0, 0, 0, 0};
-        for (uint32_t s = 0; s < 16; s++) {
-            const uint32_t k = k0 + s;
-            const uint32_t x = k < J.K ? get7(row, k, bits, eps) : 0u;
-            v[s >> 2] |= x << (8 * (s & 3));
+        if (k0 < J.K) {
+            const uint64_t* row = J.A + (uint64_t)r * rw;
+            const uint32_t w = k0 / EPS, off = k0 - w * EPS;
+            uint64_t cur = row[w] >> (off * BITS);
+            uint64_t nxt = w + 1 < rw ? row[w + 1] : 0, nxt2 = w + 2 < rw ? row[w + 2] : 0;
+            uint32_t left = EPS - off;
+#pragma unroll
+            for (uint32_t s = 0; s < 16; s++) {
+                const uint32_t x = k0 + s < J.K ? (uint32_t)(cur & MASK) : 0u;
+                v[s >> 2] |= x << (8 * (s & 3));
+                cur >>= BITS;
+                if (--left == 0) {
+                    cur = nxt;
+                    nxt = nxt2;
+                    left = EPS;
+                }
+            }
         }
         *(uint4*)(J.A8 + (uint64_t)r * J.Kp + k0) = make_uint4(v[0], v[1], v[2], v[3]);
     }
 }
 
-// B^T: one work item = one column n and 64 consecutive k; neighbouring lanes
-// take neighbouring columns, so every load instruction reads a few adjacent
-// words of one packed row.
-__global__ __launch_bounds__(256) void k_mm_unpack_bt(const MatJob* jobs, uint32_t njobs, uint32_t bits,
-                                                      uint32_t total_items)
+// B (K x N) -> int8 columns [N][Kp], through an LDS transpose. A block takes
+// MU_KT rows of k and WN whole words of every row (WN*EPS <= MU_NT columns):
+// lanes on neighbouring words load 4 rows each (coalesced row segments) and
+// write, per field, the 4 k-bytes of that column as one LDS word; then 8
+// lanes per column store its MU_KT contiguous bytes.
+template <uint32_t BITS>
+__global__ __launch_bounds__(256) void k_mm_unpack_bt(const MatJob* jobs, const MatSeg* segs)
 {
-    const uint32_t eps = 64 / bits;
-    for (uint32_t it = blockIdx.x * 256 + threadIdx.x; it < total_items; it += gridDim.x * 256) {
-        const MatJob& J = jobs[find_job(jobs, njobs, it, 1)];
-        const uint32_t local = it - J.bchunk0;
-        const uint32_t kc = local / J.N, n = local % J.N;
-        const uint32_t rw = (J.N + eps - 1) / eps;
-        const uint32_t sh = (n % eps) * bits, wi = n / eps;
-        uint32_t v[16];
+    constexpr uint32_t EPS = 64 / BITS, WN = MU_NT / EPS, NT = WN * EPS;
+    constexpr uint64_t MASK = (1ull << BITS) - 1;
+    constexpr uint32_t LS = MU_KT / 4 + 1; // LDS words per column (odd: columns spread over banks)
+    __shared__ uint32_t tr[NT * LS];
+    const MatSeg sg = segs[blockIdx.x];
+    const MatJob& J = jobs[sg.job];
+    const uint32_t ntn = (J.N + NT - 1) / NT;
+    const uint32_t kt = sg.first / ntn, nt = sg.first - kt * ntn;
+    const uint32_t k0 = kt * MU_KT, wb = nt * WN, rw = (J.N + EPS - 1) / EPS;
+    for (uint32_t task = threadIdx.x; task < WN * (MU_KT / 4); task += 256) {
+        const uint32_t q = task / WN, w = task - q * WN, gw = wb + w;
+        uint64_t x[4];
 #pragma unroll
-        for (uint32_t q = 0; q < 16; q++) v[q] = 0;
-#pragma unroll 8
-        for (uint32_t s = 0; s < 64; s++) {
-            const uint32_t k = kc * 64 + s;
-            uint32_t x = 0;
-            if (k < J.K) x = (uint32_t)((J.B[(uint64_t)k * rw + wi] >> sh) & lowmask(bits));
-            v[s >> 2] |= x << (8 * (s & 3));
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t k = k0 + 4 * q + i;
+            x[i] = (gw < rw && k < J.K) ? J.B[(uint64_t)k * rw + gw] : 0;
         }
-        uint4* dst = (uint4*)(J.B8 + (uint64_t)n * J.Kp + kc * 64);
-        dst[0] = make_uint4(v[0], v[1], v[2], v[3]);
-        dst[1] = make_uint4(v[4], v[5], v[6], v[7]);
-        dst[2] = make_uint4(v[8], v[9], v[10], v[11]);
-        dst[3] = make_uint4(v[12], v[13], v[14], v[15]);
+#pragma unroll
+        for (uint32_t f = 0; f < EPS; f++) {
+            uint32_t d = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < 4; i++) d |= (uint32_t)((x[i] >> (f * BITS)) & MASK) << (8 * i);
+            tr[(w * EPS + f) * LS + q] = d;
+        }
+    }
+    __syncthreads();
+    for (uint32_t task = threadIdx.x; task < NT * (MU_KT / 16); task += 256) {
+        const uint32_t nl = task / (MU_KT / 16), pc = task % (MU_KT / 16);
+        const uint32_t n = wb * EPS + nl;
+        if (n >= J.N) continue;
+        const uint32_t* src = tr + nl * LS + pc * 4;
+        *(uint4*)(J.B8 + (uint64_t)n * J.Kp + k0 + pc * 16) = make_uint4(src[0], src[1], src[2], src[3]);
     }
 }
 
@@ -163,29 +202,24 @@ __device__ inline MatJob uniform_job(const MatJob& j)
     u.out_bits = rfl(j.out_bits);
     u.tiles_n = rfl(j.tiles_n);
     u.tile0 = rfl(j.tile0);
-    u.achunk0 = rfl(j.achunk0);
-    u.bchunk0 = rfl(j.bchunk0);
     return u;
 }
 
 // GEMM over the batch's tile list; one 256x256 output tile per iteration.
 // 4 waves in a 2x2 arrangement, each owning 128x128 = 8x8 MFMA tiles of
-// 16x16 (256 i32 accumulators per lane). K-steps of 128 bytes are staged in
-// LDS (double-buffered, 144 KB, one block per CU); the next step's global
-// loads are in flight in registers while the current step computes.
+// 16x16 (256 i32 accumulators per lane); one block per CU (130 KB of LDS).
 __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t njobs, uint32_t total_tiles)
 {
-    // two K-step stages [A rows | B rows]; the epilogue reuses it as 128 int32 rows
-    __shared__ __attribute__((aligned(1024))) int8_t smem[MM_LDS];
+    // a ring of 4 K-step stages [A rows | B rows]; the epilogue reuses them
+    // as 4 x 32 rows of int32 C
+    __shared__ __attribute__((aligned(1024))) int8_t st0[MM_SLOT], st1[MM_SLOT], st2[MM_SLOT], st3[MM_SLOT];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wm = wave >> 1, wn = wave & 1;
     const uint32_t lr = lane & 15, lg = lane >> 4;
-    auto sA = [&](uint32_t buf) { return smem + buf * MM_STAGE; };
-    auto sB = [&](uint32_t buf) { return smem + buf * MM_STAGE + MM_BM * MM_BK; };
 
     for (uint32_t tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
         const uint32_t lt = mm_logical_tile(tile, gridDim.x, total_tiles);
-        const MatJob J = uniform_job(jobs[find_job(jobs, njobs, lt, 2)]);
+        const MatJob J = uniform_job(jobs[find_job(jobs, njobs, lt)]);
         const uint32_t local = lt - J.tile0;
         const uint32_t tiles_m = (J.M + MM_BM - 1) / MM_BM;
         const uint32_t per_group = MM_GROUP * J.tiles_n, grp = local / per_group;
@@ -193,32 +227,36 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
         const uint32_t m0 = (first + r % gsz) * MM_BM, n0 = (r / gsz) * MM_BN;
         const uint32_t steps = J.Kp / MM_BK;
 
-        // K-steps land in LDS straight from global memory (buffer loads with
-        // the LDS flag, 16 bytes per lane, no registers): each instruction
-        // fills 1 KB = 8 rows of 128 bytes, lane l the 16-byte slot l%8 of row
-        // l/8, and slot s of row R holds the row's chunk s ^ (R%8) (an XOR
-        // swizzle: the fragment reads of 8 consecutive rows then hit 8
-        // different slots). Wave w loads rows 64w..64w+63 of A and of B.
-        // Loads bounded by the tile's valid rows read zeros past M (or N).
+        // K-steps of 64 bytes land in LDS straight from global memory (buffer
+        // loads with the LDS flag, 16 bytes per lane, no registers) in a ring
+        // of MM_NSTAGE stages, MM_NSTAGE-1 steps ahead of the one computed.
+        // A stage holds 256 rows of A then 256 of B, 64 bytes each; rows pair
+        // into 128-byte lines L = R/2 of 8 slots, and chunk c of row R sits in
+        // slot (4*(R%2) + c) ^ (L%8) (an XOR swizzle: the fragment reads of 8
+        // consecutive rows hit 8 different slots). One instruction fills 1 KB
+        // = 8 lines: lane l takes line l/8, slot l%8. Wave w loads lines
+        // 32w..32w+31 of A and of B. Loads bounded by the tile's valid rows
+        // read zeros past M (or N).
         const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(J.A8 + (uint64_t)m0 * J.Kp), (short)0, (int)(min(J.M - m0, MM_BM) * J.Kp), MM_RSRC_FLAGS);
         const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(J.B8 + (uint64_t)n0 * J.Kp), (short)0, (int)(min(J.N - n0, MM_BN) * J.Kp), MM_RSRC_FLAGS);
-        const uint32_t lrow = lane >> 3;
-        const uint32_t vo = (wave * 64 + lrow) * J.Kp + ((lane & 7) ^ lrow) * 16, rs = 8 * J.Kp;
-        auto gload = [&](uint32_t ks, uint32_t buf) {
+        const uint32_t lq = (lane & 7) ^ (lane >> 3);                 // 4*(R%2) + c
+        const uint32_t lrow = 2 * (wave * 32 + (lane >> 3)) + (lq >> 2); // R for h = 0
+        const uint32_t vo = lrow * J.Kp + (lq & 3) * 16, rs = 16 * J.Kp; // 8 lines = 16 rows per h
+        auto gload = [&](uint32_t ks, int8_t* st) {
             const uint32_t ko = vo + ks * MM_BK;
 #pragma unroll
-            for (uint32_t h = 0; h < 8; h++) {
-                const uint32_t lo = (wave * 8 + h) * 1024;
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (__attribute__((address_space(3))) void*)(sA(buf) + lo),
-                                                         16, ko + h * rs, 0, 0, 0);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (__attribute__((address_space(3))) void*)(sB(buf) + lo),
-                                                         16, ko + h * rs, 0, 0, 0);
+            for (uint32_t h = 0; h < 4; h++) {
+                const uint32_t lo = (wave * 4 + h) * 1024;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (__attribute__((address_space(3))) void*)(st + lo), 16,
+                                                         ko + h * rs, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rB, (__attribute__((address_space(3))) void*)(st + MM_BM * MM_BK + lo), 16, ko + h * rs, 0, 0, 0);
             }
         };
-        // fragment reads: row R = 128*wm (or wn) + 16*i + lr, chunk 4*sub + lg
-        const uint32_t fo0 = lr * MM_BK + ((lg ^ (lr & 7)) * 16), fo1 = lr * MM_BK + (((4 + lg) ^ (lr & 7)) * 16);
+        // fragment reads: row R = 128*wm (or wn) + 16*i + lr, chunk lg
+        const uint32_t fo = (lr >> 1) * 128 + ((((lr & 1) * 4) + lg) ^ ((lr >> 1) & 7)) * 16;
 
         v4i acc[8][8];
 #pragma unroll
@@ -226,30 +264,46 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
 #pragma unroll
             for (uint32_t j = 0; j < 8; j++) acc[i][j] = v4i{0, 0, 0, 0};
 
-        __syncthreads(); // previous tile's epilogue is done with the LDS
-        gload(0, 0);
-        __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
-        __syncthreads();
-        for (uint32_t ks = 0; ks < steps; ks++) {
-            const uint32_t buf = ks & 1;
-            if (ks + 1 < steps) gload(ks + 1, buf ^ 1); // read in step ks-1, released by its barrier
+        // one K-step: wait for its loads, then refill the stage step ks-1 used
+        // and compute from `cur`. The stages are distinct LDS objects, so the
+        // compiler's own waits for the LDS-bound loads cover only `cur`.
+        auto step = [&](uint32_t ks, const int8_t* cur, int8_t* nxt) {
+            // this wave's loads of step ks have landed once at most the later
+            // steps' loads (8 instructions each) are outstanding
+            if (ks + 2 < steps) __builtin_amdgcn_s_waitcnt(MM_WAIT_VM16);
+            else if (ks + 1 < steps) __builtin_amdgcn_s_waitcnt(MM_WAIT_VM8);
+            else __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
+            // every wave's loads of step ks have landed and step ks-1 is read.
+            // A bare s_barrier: the workgroup fences of __syncthreads would
+            // also wait for the two later steps' loads. The empty asm keeps
+            // the compiler from moving LDS accesses across it (no instruction).
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (ks + 3 < steps) gload(ks + 3, nxt);
+            v4i fa[8], fb[8];
 #pragma unroll
-            for (uint32_t sub = 0; sub < MM_BK / 64; sub++) {
-                const uint32_t fo = sub ? fo1 : fo0;
-                v4i fa[8], fb[8];
-#pragma unroll
-                for (uint32_t i = 0; i < 8; i++) {
-                    fa[i] = *(const v4i*)(sA(buf) + (wm * 128 + i * 16) * MM_BK + fo);
-                    fb[i] = *(const v4i*)(sB(buf) + (wn * 128 + i * 16) * MM_BK + fo);
-                }
-#pragma unroll
-                for (uint32_t i = 0; i < 8; i++)
-#pragma unroll
-                    for (uint32_t j = 0; j < 8; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            for (uint32_t i = 0; i < 8; i++) {
+                fa[i] = *(const v4i*)(cur + (wm * 128 + i * 16) * MM_BK + fo);
+                fb[i] = *(const v4i*)(cur + MM_BM * MM_BK + (wn * 128 + i * 16) * MM_BK + fo);
             }
-            __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0); // this wave's loads of the next step have landed
-            __syncthreads();                         // ... and every wave's, and step ks is read
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++)
+#pragma unroll
+                for (uint32_t j = 0; j < 8; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        };
+
+        __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0); // the previous tile's stores are out of the count
+        __syncthreads();                         // ... and its epilogue is done with the LDS
+        gload(0, st0);
+        if (1 < steps) gload(1, st1);
+        if (2 < steps) gload(2, st2);
+        for (uint32_t ks = 0; ks < steps; ks += 4) {
+            step(ks, st0, st3);
+            if (ks + 1 < steps) step(ks + 1, st1, st0);
+            if (ks + 2 < steps) step(ks + 2, st2, st1);
+            if (ks + 3 < steps) step(ks + 3, st3, st2);
         }
 
         // ---- epilogue through LDS, one 128-row half of the tile at a time:
@@ -263,7 +317,7 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
         const bool fused = J.Cp && (16 % eps_o == 0);
         const uint32_t rw_o = (J.N + eps_o - 1) / eps_o;
         const uint64_t omask = lowmask(ob);
-        int32_t* sc = (int32_t*)smem;
+        auto slot = [&](uint32_t q) { return (int32_t*)(q == 0 ? st0 : q == 1 ? st1 : q == 2 ? st2 : st3); };
         __syncthreads(); // every wave is done reading the last K-step
 #pragma unroll
         for (uint32_t half = 0; half < 2; half++) {
@@ -274,7 +328,8 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
                     for (uint32_t j = 0; j < 8; j++)
 #pragma unroll
                         for (uint32_t r = 0; r < 4; r++)
-                            sc[(i * 16 + 4 * lg + r) * MM_C_STRIDE + wn * 128 + j * 16 + lr] = acc[i][j][r];
+                            slot(i >> 1)[((i & 1) * 16 + 4 * lg + r) * MM_C_STRIDE + wn * 128 + j * 16 + lr] =
+                                acc[i][j][r];
             }
             __syncthreads();
             const uint32_t rbase = m0 + half * 128;
@@ -282,7 +337,7 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
                 for (uint32_t t = tid; t < 128 * 64; t += 256) {
                     const uint32_t row = t >> 6, c = n0 + (t & 63) * 4, gr = rbase + row;
                     if (gr >= J.M || c >= J.N) continue;
-                    const int32_t* src = sc + row * MM_C_STRIDE + (t & 63) * 4;
+                    const int32_t* src = slot(row >> 5) + (row & 31) * MM_C_STRIDE + (t & 63) * 4;
                     int32_t* dst = J.Ci + (uint64_t)gr * J.N + c;
                     if (c + 4 <= J.N && ((uintptr_t)dst & 15) == 0) {
                         *(v4i*)dst = *(const v4i*)src;
@@ -297,7 +352,7 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
                     const uint32_t row = t >> lw, w = t & ((1u << lw) - 1), gr = rbase + row;
                     const uint32_t c0 = n0 + w * eps_o;
                     if (gr >= J.M || c0 >= J.N) continue;
-                    const int32_t* src = sc + row * MM_C_STRIDE + w * eps_o;
+                    const int32_t* src = slot(row >> 5) + (row & 31) * MM_C_STRIDE + w * eps_o;
                     uint64_t word = 0;
                     for (uint32_t q = 0; q < eps_o && c0 + q < J.N; q++)
                         word |= ((uint64_t)(uint32_t)src[q] & omask) << (q * ob);
@@ -349,15 +404,77 @@ using namespace am;
 // Workspace bytes for one matrix product (int8 copies, int32 staging, job list).
 static uint64_t mm_ws_one(uint64_t M, uint64_t N, uint64_t K)
 {
-    const uint64_t Kp = (K + MM_BK - 1) / MM_BK * MM_BK;
+    const uint64_t Kp = (K + MM_KPAD - 1) / MM_KPAD * MM_KPAD;
     return (M * Kp + 255) / 256 * 256 + (N * Kp + 255) / 256 * 256 + (M * N * 4 + 255) / 256 * 256;
+}
+
+// Unpack blocks of one product: A items / MU_A_ITEMS, plus B tiles (at least
+// 512 columns per tile for every width, so ceil(N/512) bounds the column tiles).
+static uint64_t mm_segs_bound(uint64_t M, uint64_t N, uint64_t K)
+{
+    const uint64_t Kp = (K + MM_KPAD - 1) / MM_KPAD * MM_KPAD;
+    return (M * (Kp / 16) + MU_A_ITEMS - 1) / MU_A_ITEMS + (N + 511) / 512 * (Kp / MU_KT);
+}
+
+// Header: job list, packer word offsets, unpack block table.
+static uint64_t mm_header_bytes(const uint32_t* M, const uint32_t* N, const uint32_t* K, uint32_t count)
+{
+    uint64_t segs = 0;
+    for (uint32_t i = 0; i < count; i++) segs += mm_segs_bound(M[i], N[i], K[i]);
+    return (4096 + (uint64_t)count * (sizeof(MatJob) + 4) + 8 + segs * sizeof(MatSeg) + 255) / 256 * 256;
 }
 
 extern "C" uint64_t am_matmul_workspace_bytes(const uint32_t* M, const uint32_t* N, const uint32_t* K, uint32_t count)
 {
-    uint64_t b = 4096 + (uint64_t)count * (sizeof(MatJob) + 4) + 256;
+    uint64_t b = mm_header_bytes(M, N, K, count);
     for (uint32_t i = 0; i < count; i++) b += mm_ws_one(M[i], N[i], K[i]);
     return b;
+}
+
+// Pinned host staging for the per-call header (job list, packer offsets,
+// unpack block table): one asynchronous copy per call instead of pageable
+// copies, which would block the host until the stream is idle. Two slots per
+// device; a slot is rewritten only after the copy that last used it is done.
+struct MmUpload {
+    void* host[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    uint32_t next = 0;
+};
+
+static std::mutex g_up_mu;
+static MmUpload g_up[64];
+
+// Copy `bytes` built by `fill(host_ptr)` to `dst` on stream `s` (AM_OK or AM_E_HIP).
+template <typename F>
+static int mm_upload(void* dst, size_t bytes, hipStream_t s, F fill)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AM_E_HIP;
+    std::lock_guard<std::mutex> g(g_up_mu);
+    MmUpload& u = g_up[dev];
+    const uint32_t k = u.next;
+    u.next ^= 1;
+    if (!u.done[k] && hipEventCreateWithFlags(&u.done[k], hipEventDisableTiming) != hipSuccess) return AM_E_HIP;
+    if (u.host[k] && hipEventSynchronize(u.done[k]) != hipSuccess) return AM_E_HIP; // its last copy is done
+    if (u.cap[k] < bytes) {
+        if (u.host[k]) (void)hipHostFree(u.host[k]);
+        u.host[k] = nullptr;
+        u.cap[k] = 0;
+        const size_t want = std::max<size_t>(bytes, 1 << 16);
+        if (hipHostMalloc(&u.host[k], want, hipHostMallocDefault) != hipSuccess) return AM_E_HIP;
+        u.cap[k] = want;
+    }
+    fill((uint8_t*)u.host[k]);
+    if (hipMemcpyAsync(dst, u.host[k], bytes, hipMemcpyHostToDevice, s) != hipSuccess) return AM_E_HIP;
+    return hipEventRecord(u.done[k], s) == hipSuccess ? AM_OK : AM_E_HIP;
+}
+
+template <uint32_t BITS>
+static void mm_launch_unpack(const MatJob* djobs, const MatSeg* dsegs, uint32_t na, uint32_t nb, hipStream_t s)
+{
+    if (na) hipLaunchKernelGGL(k_mm_unpack_a<BITS>, dim3(na), dim3(256), 0, s, djobs, dsegs);
+    if (nb) hipLaunchKernelGGL(k_mm_unpack_bt<BITS>, dim3(nb), dim3(256), 0, s, djobs, dsegs + na);
 }
 
 extern "C" uint32_t am_matmul_out_bits(uint32_t bits, uint32_t K)
@@ -387,8 +504,10 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
     if (ws_bytes < am_matmul_workspace_bytes(Ms.data(), Ns.data(), Ks.data(), count)) return AM_E_WORKSPACE;
     std::vector<MatJob> jobs(count);
     std::vector<uint32_t> word0(count);
-    uint8_t* cur = (uint8_t*)ws + 4096 + (uint64_t)count * (sizeof(MatJob) + 4) + 256;
-    uint64_t tiles = 0, ach = 0, bch = 0, pwords = 0;
+    uint8_t* cur = (uint8_t*)ws + mm_header_bytes(Ms.data(), Ns.data(), Ks.data(), count);
+    std::vector<MatSeg> segs_a, segs_b;
+    const uint32_t eps_i = 64 / bits, nt_b = MU_NT / eps_i * eps_i; // B-unpack tile columns
+    uint64_t tiles = 0, pwords = 0;
     bool need_pack = false;
     for (uint32_t i = 0; i < count; i++) {
         const am_matmul_desc& d = descs[i];
@@ -398,7 +517,7 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
         J.M = d.M;
         J.N = d.N;
         J.K = d.K;
-        J.Kp = (d.K + MM_BK - 1) / MM_BK * MM_BK;
+        J.Kp = (d.K + MM_KPAD - 1) / MM_KPAD * MM_KPAD;
         J.out_bits = d.out_bits ? d.out_bits : am_matmul_out_bits(bits, d.K);
         J.A8 = (int8_t*)cur;
         cur += ((uint64_t)J.M * J.Kp + 255) / 256 * 256;
@@ -414,24 +533,42 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
         cur += ((uint64_t)J.M * J.N * 4 + 255) / 256 * 256;
         J.tiles_n = (J.N + MM_BN - 1) / MM_BN;
         J.tile0 = (uint32_t)tiles;
-        J.achunk0 = (uint32_t)ach;
-        J.bchunk0 = (uint32_t)bch;
         word0[i] = (uint32_t)pwords;
         tiles += (uint64_t)((J.M + MM_BM - 1) / MM_BM) * J.tiles_n;
-        ach += (uint64_t)J.M * (J.Kp / 16);
-        bch += (uint64_t)J.N * (J.Kp / 64);
+        const uint64_t items_a = (uint64_t)J.M * (J.Kp / 16);
+        if (items_a >= (1ull << 32)) return AM_E_UNSUPPORTED_CONFIG;
+        for (uint64_t f = 0; f < items_a; f += MU_A_ITEMS) segs_a.push_back(MatSeg{i, (uint32_t)f});
+        const uint32_t tiles_b = (J.N + nt_b - 1) / nt_b * (J.Kp / MU_KT);
+        for (uint32_t t = 0; t < tiles_b; t++) segs_b.push_back(MatSeg{i, t});
         if (J.Cp && (16 % eps_o) != 0) pwords += (uint64_t)J.M * ((J.N + eps_o - 1) / eps_o);
     }
-    if (tiles >= (1ull << 32) || ach >= (1ull << 32) || bch >= (1ull << 32) || pwords >= (1ull << 32))
+    if (tiles >= (1ull << 32) || pwords >= (1ull << 32) || segs_a.size() + segs_b.size() >= (1ull << 31))
         return AM_E_UNSUPPORTED_CONFIG;
+    segs_a.insert(segs_a.end(), segs_b.begin(), segs_b.end());
     hipStream_t s = (hipStream_t)stream;
     MatJob* djobs = (MatJob*)((uint8_t*)ws + 4096);
     uint32_t* dword0 = (uint32_t*)((uint8_t*)djobs + (uint64_t)count * sizeof(MatJob));
-    if (hipMemcpyAsync(djobs, jobs.data(), count * sizeof(MatJob), hipMemcpyHostToDevice, s) != hipSuccess) return AM_E_HIP;
-    if (hipMemcpyAsync(dword0, word0.data(), count * 4, hipMemcpyHostToDevice, s) != hipSuccess) return AM_E_HIP;
+
+    MatSeg* dsegs = (MatSeg*)(((uintptr_t)(dword0 + count) + 7) & ~(uintptr_t)7);
+    const size_t o_w0 = (uint8_t*)dword0 - (uint8_t*)djobs, o_seg = (uint8_t*)dsegs - (uint8_t*)djobs;
+    const size_t hbytes = o_seg + segs_a.size() * sizeof(MatSeg);
+    const int up = mm_upload(djobs, hbytes, s, [&](uint8_t* h) {
+        memcpy(h, jobs.data(), count * sizeof(MatJob));
+        memcpy(h + o_w0, word0.data(), count * 4);
+        memcpy(h + o_seg, segs_a.data(), segs_a.size() * sizeof(MatSeg));
+    });
+    if (up != AM_OK) return up;
+    const uint32_t na = (uint32_t)(segs_a.size() - segs_b.size()), nb = (uint32_t)segs_b.size();
+    switch (bits) {
+        case 1: mm_launch_unpack<1>(djobs, dsegs, na, nb, s); break;
+        case 2: mm_launch_unpack<2>(djobs, dsegs, na, nb, s); break;
+        case 3: mm_launch_unpack<3>(djobs, dsegs, na, nb, s); break;
+        case 4: mm_launch_unpack<4>(djobs, dsegs, na, nb, s); break;
+        case 5: mm_launch_unpack<5>(djobs, dsegs, na, nb, s); break;
+        case 6: mm_launch_unpack<6>(djobs, dsegs, na, nb, s); break;
+        default: mm_launch_unpack<7>(djobs, dsegs, na, nb, s); break;
+    }
     auto grid_for = [](uint64_t items) { return (uint32_t)std::min<uint64_t>((items + 255) / 256, 65535); };
-    hipLaunchKernelGGL(k_mm_unpack_a, dim3(grid_for(ach)), dim3(256), 0, s, djobs, count, bits, (uint32_t)ach);
-    hipLaunchKernelGGL(k_mm_unpack_bt, dim3(grid_for(bch)), dim3(256), 0, s, djobs, count, bits, (uint32_t)bch);
     static uint32_t capb = 0;
     if (!capb) capb = resident_blocks(k_mm_gemm, 256, 0);
     hipLaunchKernelGGL(k_mm_gemm, dim3((uint32_t)std::min<uint64_t>(tiles, capb)), dim3(256), 0, s, djobs, count,
