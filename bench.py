@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--quick", action="store_true", help="headline only, no sweep")
+    p.add_argument("--legs", default="sweep,groupby,matmul,batched",
+                   help="comma-separated legs to run after the headline (sweep, groupby, matmul, batched)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--reps", type=int, default=10, help="launches per sweep point")
     p.add_argument("--profile-copy", action="store_true",
@@ -403,13 +405,16 @@ def main():
     achieved = gbytes(byt, avg_kernel)
     traffic = read_traffic("k_compress_w11_u32_traffic.txt")
 
-    ops = [] if args.quick else sweep(am, torch, timer, args.reps)
+    legs = set() if args.quick else set(args.legs.split(","))
+    ops = sweep(am, torch, timer, args.reps) if "sweep" in legs else []
     extra = {}
-    if not args.quick:
+    if "groupby" in legs:
         print("group-by", file=sys.stderr, flush=True)
         ops += groupby_leg(am, torch, timer, args.reps)
+    if "matmul" in legs:
         print("matmul", file=sys.stderr, flush=True)
         ops += matmul_leg(am, torch, timer, args.reps)
+    if "batched" in legs:
         print("batched", file=sys.stderr, flush=True)
         extra["batched"] = batched_leg(am, torch, args.reps)
     # measured copy peak on this box: a plain 1 GiB device-to-device copy
