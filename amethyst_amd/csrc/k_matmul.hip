@@ -1,5 +1,5 @@
 // k_matmul.hip -- exact integer matmul of bit-packed <= 7-bit matrices on
-// the int8 matrix cores (v_mfma_i32_16x16x64_i8, exact i32 accumulation).
+// the int8 matrix cores (v_mfma_i32_32x32x32_i8, exact i32 accumulation).
 //
 // Reference: the packed-matrix layout helpers only -- matrix_mul_calc_rowcolcnt
 // and get_matrix_element (src/common_code.cuh:65-86,118-134): row-major,
@@ -32,17 +32,18 @@
 namespace am {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t MM_BM = 256, MM_BN = 256, MM_BK = 64, MM_GROUP = 8, MM_NSTAGE = 4;
-constexpr uint32_t MM_KPAD = 128; // K padding of the int8 copies (whole B-unpack tiles)
+constexpr uint32_t MM_BM = 256, MM_BN = 256, MM_BK = 128, MM_GROUP = 8;
+constexpr uint32_t MM_KPAD = 128; // K padding of the int8 copies (whole K-steps and B-unpack tiles)
 constexpr int MM_RSRC_FLAGS = 0x00020000;        // buffer descriptor word 3: 32-bit data format
 constexpr uint32_t MM_STAGE = (MM_BM + MM_BN) * MM_BK; // bytes of one K-step stage (unpadded rows)
 constexpr uint32_t MM_C_STRIDE = 260;            // int32 per staged C row (4*stride = 16 mod 32 banks)
-constexpr uint32_t MM_SLOT = 32 * MM_C_STRIDE * 4; // one stage; in the epilogue 32 rows of C
+constexpr uint32_t MM_SLOT = 64 * MM_C_STRIDE * 4; // one stage; in the epilogue 64 rows of C
 static_assert(MM_SLOT >= MM_STAGE, "a stage fits its slot");
-// s_waitcnt vmcnt(n) with the other counters unconstrained (vmcnt bits 3:0 and 15:14)
-constexpr unsigned MM_WAIT_VM0 = 0x0F70, MM_WAIT_VM8 = 0x0F78, MM_WAIT_VM16 = 0x4F70;
-static_assert(MM_NSTAGE == 4, "the vmcnt waits assume 3 steps in flight, 8 loads per step and wave");
+// s_waitcnt vmcnt(0) with the other counters unconstrained (vmcnt bits 3:0 and 15:14)
+constexpr unsigned MM_WAIT_VM0 = 0x0F70;
+constexpr uint32_t MM_THREADS = 512; // 8 waves: 4 x 64 rows by 2 x 128 columns of the tile
 
 struct MatJob {
     const uint64_t* A; // packed M x K
@@ -206,16 +207,32 @@ __device__ inline MatJob uniform_job(const MatJob& j)
 }
 
 // GEMM over the batch's tile list; one 256x256 output tile per iteration.
-// 4 waves in a 2x2 arrangement, each owning 128x128 = 8x8 MFMA tiles of
-// 16x16 (256 i32 accumulators per lane); one block per CU (130 KB of LDS).
-__global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t njobs, uint32_t total_tiles)
+// 4 waves in a 2x2 arrangement, each owning 128x128 = 4x4 MFMA tiles of
+// 32x32 (v_mfma_i32_32x32x32_i8; 16 i32 accumulators per tile and lane, 256
+// in all); one block per CU.
+//
+// K-steps of 128 bytes land in LDS straight from global memory (buffer loads
+// with the LDS flag, 16 bytes per lane, no registers), one step ahead, in two
+// stage arrays used alternately. A stage holds 256 rows of A then 256 of B,
+// 128 bytes each; chunk c (16 bytes) of row R sits in slot c ^ (R % 8) (an XOR
+// swizzle: the fragment reads of 8 consecutive rows hit 8 different slots).
+// One load instruction fills 1 KB = 8 rows: lane l takes row l/8, slot l%8.
+// Wave w loads rows 64w..64w+63 of A and of B. Loads bounded by the tile's
+// valid rows read zeros past M (or N).
+//
+// The stages are distinct LDS objects and the K loop is unrolled by 2, so the
+// compiler knows the fragment reads of one stage do not alias the loads in
+// flight into the other and inserts no wait for them.
+__global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, uint32_t njobs, uint32_t total_tiles)
 {
-    // a ring of 4 K-step stages [A rows | B rows]; the epilogue reuses them
-    // as 4 x 32 rows of int32 C
-    __shared__ __attribute__((aligned(1024))) int8_t st0[MM_SLOT], st1[MM_SLOT], st2[MM_SLOT], st3[MM_SLOT];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t wm = wave >> 1, wn = wave & 1;
-    const uint32_t lr = lane & 15, lg = lane >> 4;
+    // two K-step stages [A rows | B rows]; the epilogue reuses them as 2 x 64
+    // rows of int32 C
+    __shared__ __attribute__((aligned(1024))) int8_t st0[MM_SLOT], st1[MM_SLOT];
+    // wave index as a scalar: the LDS destinations of the loads are then
+    // scalar arithmetic (M0) rather than per-lane values
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = rfl(tid >> 6);
+    const uint32_t wm = wave >> 1, wn = wave & 1; // wm: 64-row band (0..3), wn: 128-column half
+    const uint32_t l32 = lane & 31, lh = lane >> 5;
 
     for (uint32_t tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
         const uint32_t lt = mm_logical_tile(tile, gridDim.x, total_tiles);
@@ -227,23 +244,12 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
         const uint32_t m0 = (first + r % gsz) * MM_BM, n0 = (r / gsz) * MM_BN;
         const uint32_t steps = J.Kp / MM_BK;
 
-        // K-steps of 64 bytes land in LDS straight from global memory (buffer
-        // loads with the LDS flag, 16 bytes per lane, no registers) in a ring
-        // of MM_NSTAGE stages, MM_NSTAGE-1 steps ahead of the one computed.
-        // A stage holds 256 rows of A then 256 of B, 64 bytes each; rows pair
-        // into 128-byte lines L = R/2 of 8 slots, and chunk c of row R sits in
-        // slot (4*(R%2) + c) ^ (L%8) (an XOR swizzle: the fragment reads of 8
-        // consecutive rows hit 8 different slots). One instruction fills 1 KB
-        // = 8 lines: lane l takes line l/8, slot l%8. Wave w loads lines
-        // 32w..32w+31 of A and of B. Loads bounded by the tile's valid rows
-        // read zeros past M (or N).
         const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(J.A8 + (uint64_t)m0 * J.Kp), (short)0, (int)(min(J.M - m0, MM_BM) * J.Kp), MM_RSRC_FLAGS);
         const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(J.B8 + (uint64_t)n0 * J.Kp), (short)0, (int)(min(J.N - n0, MM_BN) * J.Kp), MM_RSRC_FLAGS);
-        const uint32_t lq = (lane & 7) ^ (lane >> 3);                 // 4*(R%2) + c
-        const uint32_t lrow = 2 * (wave * 32 + (lane >> 3)) + (lq >> 2); // R for h = 0
-        const uint32_t vo = lrow * J.Kp + (lq & 3) * 16, rs = 16 * J.Kp; // 8 lines = 16 rows per h
+        const uint32_t lrow = lane >> 3;
+        const uint32_t vo = (wave * 32 + lrow) * J.Kp + ((lane & 7) ^ lrow) * 16, rs = 8 * J.Kp;
         auto gload = [&](uint32_t ks, int8_t* st) {
             const uint32_t ko = vo + ks * MM_BK;
 #pragma unroll
@@ -255,89 +261,88 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
                     rB, (__attribute__((address_space(3))) void*)(st + MM_BM * MM_BK + lo), 16, ko + h * rs, 0, 0, 0);
             }
         };
-        // fragment reads: row R = 128*wm (or wn) + 16*i + lr, chunk lg
-        const uint32_t fo = (lr >> 1) * 128 + ((((lr & 1) * 4) + lg) ^ ((lr >> 1) & 7)) * 16;
+        // fragment reads: row R = 128*wm (or wn) + 32*i + l32, k chunk 2*sub + lh
+        // (k = 32*sub .. 32*sub+31 of the step; both operands map k to lanes alike)
+        const uint32_t fbase = l32 * MM_BK, fx = l32 & 7;
 
-        v4i acc[8][8];
+        v16i acc[2][4];
 #pragma unroll
-        for (uint32_t i = 0; i < 8; i++)
+        for (uint32_t i = 0; i < 2; i++)
 #pragma unroll
-            for (uint32_t j = 0; j < 8; j++) acc[i][j] = v4i{0, 0, 0, 0};
+            for (uint32_t j = 0; j < 4; j++) acc[i][j] = v16i{};
 
-        // one K-step: wait for its loads, then refill the stage step ks-1 used
-        // and compute from `cur`. The stages are distinct LDS objects, so the
-        // compiler's own waits for the LDS-bound loads cover only `cur`.
+        // one K-step: refill the other stage with step ks+1, compute from
+        // `cur`, then wait for this wave's loads and meet the other waves
         auto step = [&](uint32_t ks, const int8_t* cur, int8_t* nxt) {
-            // this wave's loads of step ks have landed once at most the later
-            // steps' loads (8 instructions each) are outstanding
-            if (ks + 2 < steps) __builtin_amdgcn_s_waitcnt(MM_WAIT_VM16);
-            else if (ks + 1 < steps) __builtin_amdgcn_s_waitcnt(MM_WAIT_VM8);
-            else __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
-            // every wave's loads of step ks have landed and step ks-1 is read.
-            // A bare s_barrier: the workgroup fences of __syncthreads would
-            // also wait for the two later steps' loads. The empty asm keeps
-            // the compiler from moving LDS accesses across it (no instruction).
+            if (ks + 1 < steps) gload(ks + 1, nxt);
+#pragma unroll
+            for (uint32_t sub = 0; sub < 4; sub++) {
+                const uint32_t fo = fbase + (((2 * sub + lh) ^ fx) * 16);
+                v4i fa[2], fb[4];
+#pragma unroll
+                for (uint32_t i = 0; i < 2; i++) fa[i] = *(const v4i*)(cur + (wm * 64 + i * 32) * MM_BK + fo);
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++)
+                    fb[j] = *(const v4i*)(cur + MM_BM * MM_BK + (wn * 128 + j * 32) * MM_BK + fo);
+#pragma unroll
+                for (uint32_t i = 0; i < 2; i++)
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0); // keep each sub-step's fragments to itself (register pressure)
+            }
+            // step ks+1 has landed for this wave; after the barrier for all
+            // waves, and every wave is done reading `cur`. A bare s_barrier
+            // (no fences; the wait above is the one needed); the empty asm
+            // keeps the compiler from moving LDS accesses across it.
+            __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
             asm volatile("" ::: "memory");
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
-            if (ks + 3 < steps) gload(ks + 3, nxt);
-            v4i fa[8], fb[8];
-#pragma unroll
-            for (uint32_t i = 0; i < 8; i++) {
-                fa[i] = *(const v4i*)(cur + (wm * 128 + i * 16) * MM_BK + fo);
-                fb[i] = *(const v4i*)(cur + MM_BM * MM_BK + (wn * 128 + i * 16) * MM_BK + fo);
-            }
-#pragma unroll
-            for (uint32_t i = 0; i < 8; i++)
-#pragma unroll
-                for (uint32_t j = 0; j < 8; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
         };
 
         __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0); // the previous tile's stores are out of the count
         __syncthreads();                         // ... and its epilogue is done with the LDS
         gload(0, st0);
-        if (1 < steps) gload(1, st1);
-        if (2 < steps) gload(2, st2);
-        for (uint32_t ks = 0; ks < steps; ks += 4) {
-            step(ks, st0, st3);
+        __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
+        __syncthreads();
+        for (uint32_t ks = 0; ks < steps; ks += 2) {
+            step(ks, st0, st1);
             if (ks + 1 < steps) step(ks + 1, st1, st0);
-            if (ks + 2 < steps) step(ks + 2, st2, st1);
-            if (ks + 3 < steps) step(ks + 3, st3, st2);
         }
 
         // ---- epilogue through LDS, one 128-row half of the tile at a time:
         // the two waves owning the half write their accumulators (lane holds
-        // C[row0 + 4*lg + r][col0 + lr], r < 4) as int32 rows, then all
-        // threads store whole rows: int32 C in 16-byte pieces and/or packed
-        // words (when floor(64/out_bits) divides 16, words never straddle
-        // tiles; other widths are packed later from the int32 copy)
+        // C[row0 + 8*(q/4) + 4*(lane/32) + q%4][col0 + lane%32], q < 16) as
+        // int32 rows, then all threads store whole rows: int32 C in 16-byte
+        // pieces and/or packed words (when floor(64/out_bits) divides 16,
+        // words never straddle tiles; other widths are packed later from the
+        // int32 copy)
         const uint32_t ob = J.out_bits;
         const uint32_t eps_o = 64 / ob;
         const bool fused = J.Cp && (16 % eps_o == 0);
         const uint32_t rw_o = (J.N + eps_o - 1) / eps_o;
         const uint64_t omask = lowmask(ob);
-        auto slot = [&](uint32_t q) { return (int32_t*)(q == 0 ? st0 : q == 1 ? st1 : q == 2 ? st2 : st3); };
-        __syncthreads(); // every wave is done reading the last K-step
+        auto slot = [&](uint32_t q) { return (int32_t*)(q == 0 ? st0 : st1); };
 #pragma unroll
         for (uint32_t half = 0; half < 2; half++) {
-            if (wm == half) {
+            if ((wm >> 1) == half) {
 #pragma unroll
-                for (uint32_t i = 0; i < 8; i++)
+                for (uint32_t i = 0; i < 2; i++)
 #pragma unroll
-                    for (uint32_t j = 0; j < 8; j++)
+                    for (uint32_t j = 0; j < 4; j++)
 #pragma unroll
-                        for (uint32_t r = 0; r < 4; r++)
-                            slot(i >> 1)[((i & 1) * 16 + 4 * lg + r) * MM_C_STRIDE + wn * 128 + j * 16 + lr] =
-                                acc[i][j][r];
+                        for (uint32_t q = 0; q < 16; q++)
+                            slot(wm & 1)[(i * 32 + 8 * (q >> 2) + 4 * lh + (q & 3)) * MM_C_STRIDE + wn * 128 + j * 32 +
+                                         l32] = acc[i][j][q];
             }
             __syncthreads();
             const uint32_t rbase = m0 + half * 128;
             if (J.Ci) {
-                for (uint32_t t = tid; t < 128 * 64; t += 256) {
+                for (uint32_t t = tid; t < 128 * 64; t += MM_THREADS) {
                     const uint32_t row = t >> 6, c = n0 + (t & 63) * 4, gr = rbase + row;
                     if (gr >= J.M || c >= J.N) continue;
-                    const int32_t* src = slot(row >> 5) + (row & 31) * MM_C_STRIDE + (t & 63) * 4;
+                    const int32_t* src = slot(row >> 6) + (row & 63) * MM_C_STRIDE + (t & 63) * 4;
                     int32_t* dst = J.Ci + (uint64_t)gr * J.N + c;
                     if (c + 4 <= J.N && ((uintptr_t)dst & 15) == 0) {
                         *(v4i*)dst = *(const v4i*)src;
@@ -348,11 +353,11 @@ __global__ __launch_bounds__(256, 1) void k_mm_gemm(const MatJob* jobs, uint32_t
             }
             if (fused) {
                 const uint32_t lw = 31 - __builtin_clz(256 / eps_o); // log2 words per tile row
-                for (uint32_t t = tid; t < (128u << lw); t += 256) {
+                for (uint32_t t = tid; t < (128u << lw); t += MM_THREADS) {
                     const uint32_t row = t >> lw, w = t & ((1u << lw) - 1), gr = rbase + row;
                     const uint32_t c0 = n0 + w * eps_o;
                     if (gr >= J.M || c0 >= J.N) continue;
-                    const int32_t* src = slot(row >> 5) + (row & 31) * MM_C_STRIDE + w * eps_o;
+                    const int32_t* src = slot(row >> 6) + (row & 63) * MM_C_STRIDE + w * eps_o;
                     uint64_t word = 0;
                     for (uint32_t q = 0; q < eps_o && c0 + q < J.N; q++)
                         word |= ((uint64_t)(uint32_t)src[q] & omask) << (q * ob);
@@ -570,8 +575,8 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
     }
     auto grid_for = [](uint64_t items) { return (uint32_t)std::min<uint64_t>((items + 255) / 256, 65535); };
     static uint32_t capb = 0;
-    if (!capb) capb = resident_blocks(k_mm_gemm, 256, 0);
-    hipLaunchKernelGGL(k_mm_gemm, dim3((uint32_t)std::min<uint64_t>(tiles, capb)), dim3(256), 0, s, djobs, count,
+    if (!capb) capb = resident_blocks(k_mm_gemm, MM_THREADS, 0);
+    hipLaunchKernelGGL(k_mm_gemm, dim3((uint32_t)std::min<uint64_t>(tiles, capb)), dim3(MM_THREADS), 0, s, djobs, count,
                        (uint32_t)tiles);
     if (need_pack)
         hipLaunchKernelGGL(k_mm_pack_rows, dim3(grid_for(pwords)), dim3(256), 0, s, djobs, count, dword0,
