@@ -11,9 +11,9 @@
 //   1. unpack A (M x K) to int8 rows [M][Kp] and B (K x N) to int8 columns
 //      [N][Kp] (Kp = K rounded up to 128, zero-padded): 7-bit values are
 //      non-negative in int8, so the signed int8 MFMA is exact;
-//   2. GEMM: 256x256 block tiles, K-steps of 64 loaded straight into a
-//      4-stage LDS ring (3 steps in flight); each of 4 waves owns a 128x128
-//      quadrant = 8x8 MFMA tiles of 16x16 (i32 accumulators).
+//   2. GEMM: 256x256 block tiles, K-steps of 128 loaded straight into LDS
+//      one step ahead; each of 8 waves owns 64x128 = 2x4 MFMA tiles of 32x32
+//      (i32 accumulators).
 //      Both operands use the same k-to-slot mapping per lane, so the sum over
 //      k is exact whatever order the matrix core visits the slots in;
 //   3. epilogue: C as row-major int32 and/or re-packed as a row-padded slab
@@ -207,9 +207,10 @@ __device__ inline MatJob uniform_job(const MatJob& j)
 }
 
 // GEMM over the batch's tile list; one 256x256 output tile per iteration.
-// 4 waves in a 2x2 arrangement, each owning 128x128 = 4x4 MFMA tiles of
-// 32x32 (v_mfma_i32_32x32x32_i8; 16 i32 accumulators per tile and lane, 256
-// in all); one block per CU.
+// 8 waves, 4 bands of 64 rows by 2 halves of 128 columns, each owning
+// 64x128 = 2x4 MFMA tiles of 32x32 (v_mfma_i32_32x32x32_i8; 16 i32
+// accumulators per tile and lane, 128 in all); one block per CU, 2 waves per
+// SIMD.
 //
 // K-steps of 128 bytes land in LDS straight from global memory (buffer loads
 // with the LDS flag, 16 bytes per lane, no registers), one step ahead, in two
@@ -217,7 +218,7 @@ __device__ inline MatJob uniform_job(const MatJob& j)
 // 128 bytes each; chunk c (16 bytes) of row R sits in slot c ^ (R % 8) (an XOR
 // swizzle: the fragment reads of 8 consecutive rows hit 8 different slots).
 // One load instruction fills 1 KB = 8 rows: lane l takes row l/8, slot l%8.
-// Wave w loads rows 64w..64w+63 of A and of B. Loads bounded by the tile's
+// Wave w loads rows 32w..32w+31 of A and of B. Loads bounded by the tile's
 // valid rows read zeros past M (or N).
 //
 // The stages are distinct LDS objects and the K loop is unrolled by 2, so the
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
         }
 
         // ---- epilogue through LDS, one 128-row half of the tile at a time:
-        // the two waves owning the half write their accumulators (lane holds
+        // the four waves owning the half write their accumulators (lane holds
         // C[row0 + 8*(q/4) + 4*(lane/32) + q%4][col0 + lane%32], q < 16) as
         // int32 rows, then all threads store whole rows: int32 C in 16-byte
         // pieces and/or packed words (when floor(64/out_bits) divides 16,
