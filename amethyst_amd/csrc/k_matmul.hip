@@ -515,8 +515,14 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
     const uint32_t eps_i = 64 / bits, nt_b = MU_NT / eps_i * eps_i; // B-unpack tile columns
     uint64_t tiles = 0, pwords = 0;
     bool need_pack = false;
+    // jobs in order of decreasing K (the work per GEMM tile): the persistent
+    // grid takes tiles in job order, so the longest tiles start first and
+    // the short ones fill the tail (results are per job, order-independent)
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return descs[x].K > descs[y].K; });
     for (uint32_t i = 0; i < count; i++) {
-        const am_matmul_desc& d = descs[i];
+        const am_matmul_desc& d = descs[order[i]];
         MatJob& J = jobs[i];
         J.A = d.A;
         J.B = d.B;
