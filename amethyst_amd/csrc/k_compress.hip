@@ -272,7 +272,7 @@ constexpr uint32_t CW_MASKW = 256; // output mask words per tile (4*R*eps)
 constexpr uint32_t CD_RMAX = 8;
 constexpr uint32_t CD_WAVES = 4;
 constexpr uint32_t CD_THREADS = CD_WAVES * 64;
-constexpr uint32_t CD_STAGE = 20480;                    // LDS bytes: input words, then compacted values
+constexpr uint32_t CD_STAGE = 19456; // LDS bytes: input words, then compacted values (8 blocks per CU)
 constexpr uint32_t CD_STAGE_C = 16384; // container outputs: the same, compacted values in passes (8 blocks per CU)
 
 // Copy values s_vals[0, cnt) to out[P, P + cnt) (capacity-checked), four
@@ -314,11 +314,13 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
         if (P + i < cap) out[P + i] = (C)s_vals[i];
 }
 
-// container outputs of a count-only scan fit 8 waves per SIMD (8 blocks per
-// CU, the most 4-wave blocks a CU holds): ask the compiler to keep them there
+// container outputs of a count-only scan and packed outputs fit 8 waves per
+// SIMD (8 blocks per CU, the most 4-wave blocks a CU holds; packed outputs
+// spill a few registers for it and still gain 3-8 %): ask the compiler to
+// keep them there
 template <int OK, bool OVF>
 struct CdWaves {
-    static constexpr int V = (OK == K_STRIDED && !OVF) ? 8 : 1;
+    static constexpr int V = (OK != K_STRIDED || !OVF) ? 8 : 1;
 };
 
 template <int OK, bool PRED, bool OVF, int VB, bool WIDE>
