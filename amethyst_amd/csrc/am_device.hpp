@@ -45,6 +45,16 @@ struct Pred {
     __device__ inline bool sel32(uint32_t v) const { return ((v - (uint32_t)lo) <= (uint32_t)span) != (inv != 0); }
 };
 
+// Streaming stores for outputs written once and not read again by the
+// kernel (the non-temporal hint keeps them from displacing cached inputs and
+// look-back state; 4 % on the 1 GB compaction). LD_NT is the matching cache
+// policy for global->LDS loads of inputs read once.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+template <typename T> __device__ inline void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+constexpr int LD_NT = 2; // cache-policy operand: non-temporal
+
 // 32-bit quotient idx/eps for idx < 2^24 and eps in 2..64 (multiply-high by
 // floor(2^32/eps)+1: the error term stays below 2^-8 < 1/eps).
 __device__ inline uint32_t div_eps(uint32_t idx, const Col& c)

@@ -200,7 +200,7 @@ __device__ inline void tile_write(const CompressArgs& a, const V* s_vals, uint64
                     word |= v << sh;
                 }
             }
-            a.out.p[kf + j] = word;
+            st_nt(a.out.p + kf + j, word);
         }
     }
     else {
@@ -233,7 +233,7 @@ __device__ inline void tile_write(const CompressArgs& a, const V* s_vals, uint64
             }
             const uint64_t b0 = (kf + j) * 64;
             if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-            a.out.p[kf + j] = word;
+            st_nt(a.out.p + kf + j, word);
         }
     }
 }
@@ -286,21 +286,21 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
     if (!a.out_vec) head = cnt; // unaligned output buffer: element stores only
     if (head > cnt) head = cnt;
     for (uint32_t i = tid; i < head; i += CD_THREADS)
-        if (P + i < cap) out[P + i] = (C)s_vals[i];
+        if (P + i < cap) st_nt(out + P + i, (C)s_vals[i]);
     const uint32_t nq = (cnt - head) / 4;
     for (uint32_t q = tid; q < nq; q += CD_THREADS) {
         const uint32_t i0 = head + 4 * q;
         const uint64_t r = P + i0;
         const C v0 = (C)s_vals[i0], v1 = (C)s_vals[i0 + 1], v2 = (C)s_vals[i0 + 2], v3 = (C)s_vals[i0 + 3];
         if (r + 4 <= cap) {
-            if constexpr (sizeof(C) == 4) *(uint4*)(out + r) = make_uint4(v0, v1, v2, v3);
+            if constexpr (sizeof(C) == 4) st_nt((u32x4*)(out + r), u32x4{(uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3});
             else if constexpr (sizeof(C) == 2)
-                *(uint2*)(out + r) = make_uint2((uint32_t)v0 | ((uint32_t)v1 << 16), (uint32_t)v2 | ((uint32_t)v3 << 16));
+                st_nt((u32x2*)(out + r), u32x2{(uint32_t)v0 | ((uint32_t)v1 << 16), (uint32_t)v2 | ((uint32_t)v3 << 16)});
             else if constexpr (sizeof(C) == 1)
-                *(uint32_t*)(out + r) = (uint32_t)v0 | ((uint32_t)v1 << 8) | ((uint32_t)v2 << 16) | ((uint32_t)v3 << 24);
+                st_nt((uint32_t*)(out + r), (uint32_t)v0 | ((uint32_t)v1 << 8) | ((uint32_t)v2 << 16) | ((uint32_t)v3 << 24));
             else {
-                *(ulonglong2*)(out + r) = make_ulonglong2(v0, v1);
-                *(ulonglong2*)(out + r + 2) = make_ulonglong2(v2, v3);
+                st_nt((u64x2*)(out + r), u64x2{(uint64_t)v0, (uint64_t)v1});
+                st_nt((u64x2*)(out + r + 2), u64x2{(uint64_t)v2, (uint64_t)v3});
             }
         }
         else {
@@ -311,7 +311,7 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
         }
     }
     for (uint32_t i = head + 4 * nq + tid; i < cnt; i += CD_THREADS)
-        if (P + i < cap) out[P + i] = (C)s_vals[i];
+        if (P + i < cap) st_nt(out + P + i, (C)s_vals[i]);
 }
 
 // container outputs of a count-only scan and packed outputs fit 8 waves per
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
                 const uint32_t wl = wl0 + j * 64 + 2 * lane;
                 const uint64_t gw = w0 + wl;
                 if (wl < wend) {
-                    if (gw + 2 <= nw) __builtin_amdgcn_global_load_lds(a.in.p + gw, (lds_ptr)(s_x + wl0 + j * 64), 16, 0, 0);
+                    if (gw + 2 <= nw) __builtin_amdgcn_global_load_lds(a.in.p + gw, (lds_ptr)(s_x + wl0 + j * 64), 16, 0, LD_NT);
                     else if (gw < nw) s_x[wl] = a.in.p[gw];
                 }
             }
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
             for (uint32_t j = 0; j < 2 * R; j++) { // 32 words per instruction, 4 bytes per lane
                 const uint32_t dl = 2 * wl0 + j * 64 + lane;
                 if (2 * w0 + dl < 2 * nw)
-                    __builtin_amdgcn_global_load_lds(src + 2 * w0 + dl, (lds_ptr)((uint32_t*)s_x + 2 * wl0 + j * 64), 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds(src + 2 * w0 + dl, (lds_ptr)((uint32_t*)s_x + 2 * wl0 + j * 64), 4, 0, LD_NT);
             }
         }
         if (!PRED) {
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
                     const uint64_t m = eps == 1 ? ballot(gg & 1)
                                                 : segment_or((uint64_t)gg << __umul24(lane & (S - 1), eps), S);
                     const uint64_t mi = mw0 + (uint64_t)ci * eps + lane / S;
-                    if ((lane & (S - 1)) == S - 1 && mi < nmw) a.mask_out[mi] = m;
+                    if ((lane & (S - 1)) == S - 1 && mi < nmw) st_nt(a.mask_out + mi, m);
                 }
                 else // the selection bits, into this wave's own (consumed) input words
                     ((uint32_t*)(s_x + wave * 64 * R))[r * 64 + lane] = gg;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
                 }
                 if (sh < 64) m |= (uint64_t)sg[w] << sh;
                 const uint64_t mi = mw0 + (uint64_t)wave * nwm + lane;
-                if (mi < nmw) a.mask_out[mi] = m; // bits past n are zero (cleared above)
+                if (mi < nmw) st_nt(a.mask_out + mi, m); // bits past n are zero (cleared above)
             }
         }
     }
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
     if (!PRED && a.mask_out && tid < NMW && mw0 + tid < nmw) {
         uint64_t m = s_mask[tid];
         if (mw0 + tid == nmw - 1 && (n & 63)) m &= lowmask((uint32_t)(n & 63)); // input mask bits past n
-        a.mask_out[mw0 + tid] = m;
+        st_nt(a.mask_out + mw0 + tid, m);
     }
     // wave 0 publishes the tile's aggregate before the block compacts, and
     // looks back after. Packed outputs carry the tile's window (the last 63
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
     }
     if (a.mask_out) {
         const uint64_t w = (wb >> 6) + lane;
-        if (lane < CE_J && w < nmw) a.mask_out[w] = mine;
+        if (lane < CE_J && w < nmw) st_nt(a.mask_out + w, mine);
     }
     if (lane == 0) {
         s_wcnt[wave] = cnt;

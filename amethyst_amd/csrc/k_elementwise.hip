@@ -65,10 +65,10 @@ __global__ __launch_bounds__(256) void k_elementwise(Col a, Col b, Col out)
             if (u >= n) break;
             uint64_t v = value((uint32_t)(u - t0));
             switch (out.cbytes) {
-                case 1: ((uint8_t*)out.p)[u] = (uint8_t)v; break;
-                case 2: ((uint16_t*)out.p)[u] = (uint16_t)v; break;
-                case 4: ((uint32_t*)out.p)[u] = (uint32_t)v; break;
-                default: out.p[u] = v;
+                case 1: st_nt((uint8_t*)out.p + u, (uint8_t)v); break;
+                case 2: st_nt((uint16_t*)out.p + u, (uint16_t)v); break;
+                case 4: st_nt((uint32_t*)out.p + u, (uint32_t)v); break;
+                default: st_nt(out.p + u, v);
             }
         }
         else if (OK == K_SLAB) {
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_elementwise(Col a, Col b, Col out)
             const uint32_t cnt = (uint32_t)(n - e0 < out.eps ? n - e0 : out.eps);
             uint64_t word = 0;
             for (uint32_t s = 0; s < cnt; s++) word |= value((uint32_t)(e0 - t0) + s) << (s * bo);
-            out.p[u] = word;
+            st_nt(out.p + u, word);
         }
         else {
             const uint64_t b0 = u * 64;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_elementwise(Col a, Col b, Col out)
                 uint64_t v = value((uint32_t)(e - t0));
                 word |= pos >= 0 ? (v << pos) : (v >> (-pos));
             }
-            out.p[u] = word;
+            st_nt(out.p + u, word);
         }
     }
 }
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
     const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
     uint64_t word = 0;
     if (ALIGNED) {
-        const uint64_t x = g.a.p[q], y = g.b.p[q];
+        const uint64_t x = __builtin_nontemporal_load(g.a.p + q), y = __builtin_nontemporal_load(g.b.p + q);
         for (uint32_t j = 0, si = 0, so = 0; j < eo; j++, si += fi, so += fo) {
             const uint64_t r = binop<OP>((x >> si) & im, (y >> si) & im) & om;
             word |= (j < cnt ? r : 0) << so;
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
             word |= (j < cnt ? r : 0) << so;
         }
     }
-    g.out.p[q] = word;
+    st_nt(g.out.p + q, word);
 }
 
 template <int OP>
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_binary_stream(BinStream g)
             const uint64_t r = binop<OP>((uint64_t)va, (uint64_t)vb) & om;
             word |= pos >= 0 ? (r << pos) : (r >> (-pos));
         }
-        g.out.p[q0 + qr] = word;
+        st_nt(g.out.p + q0 + qr, word);
     }
 }
 

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_
             if (lane == j) mine = b;
         }
         const uint64_t w = (wbase >> 6) + lane;
-        if (lane < FT_J && w < words) mask_out[w] = mine;
+        if (lane < FT_J && w < words) st_nt(mask_out + w, mine);
 #pragma unroll
         for (uint32_t j = 0; j < FT_J; j++) f[j] = nf[j];
     }
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_filter_stream(Col in, Pred pr, uint64_t
         if (lane == j) mine = m;
     }
     const uint64_t mw = (t0 >> 6) + wave * FS_J + lane;
-    if (lane < FS_J && mw < ((n + 63) >> 6)) mask_out[mw] = mine;
+    if (lane < FS_J && mw < ((n + 63) >> 6)) st_nt(mask_out + mw, mine);
 }
 
 // Word-per-lane filter for slab and container inputs: a block owns a tile
@@ -170,12 +170,12 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
         const uint64_t cmw = mw0 + (uint64_t)ci * eps; // first mask word of the chunk
         if (eps == 1) { // the chunk's mask word is the ballot
             const uint64_t m = ballot(gg & 1);
-            if (lane == 0 && cmw < nmw) a.mask_out[cmw] = m;
+            if (lane == 0 && cmw < nmw) st_nt(a.mask_out + cmw, m);
         }
         else if (!lds) {
             const uint64_t m = segment_or((uint64_t)gg << __umul24(lane & (S - 1), eps), S);
             const uint64_t mi = cmw + lane / S;
-            if ((lane & (S - 1)) == S - 1 && mi < nmw) a.mask_out[mi] = m;
+            if ((lane & (S - 1)) == S - 1 && mi < nmw) st_nt(a.mask_out + mi, m);
         }
         else if (gg) {
             const uint32_t p = ci * 64 * eps + leps, w = p >> 6, off = p & 63;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
     if (!lds) return;
     __syncthreads();
     for (uint32_t i = tid; i < NMW; i += 256)
-        if (mw0 + i < nmw) a.mask_out[mw0 + i] = s_mask[i];
+        if (mw0 + i < nmw) st_nt(a.mask_out + mw0 + i, s_mask[i]);
 }
 
 // ---- sum: lane-per-word decode, per-block partials, then one final block.
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_sum_swar(SumSwar g, uint64_t* partial)
 #pragma unroll
         for (uint32_t u = 0; u < SUM_UNROLL; u++) {
             const uint64_t q = q0 + u * 256;
-            uint64_t x = q < g.words ? g.p[q] : 0;
+            uint64_t x = q < g.words ? __builtin_nontemporal_load(g.p + q) : 0;
             if (q == g.words - 1) x &= lowmask((uint32_t)(g.n_last * g.fw)); // fields past n
             wv[u] = x;
         }
