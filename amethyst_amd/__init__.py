@@ -94,6 +94,7 @@ def lib():
             "am_gen_mask_bernoulli": ([u64, ctypes.c_double, vp, u64, vp], i32),
             "am_gen_mask_runs": ([u64, ctypes.c_double, ctypes.c_double, vp, u64, vp], i32),
             "am_gen_sorted_noise": ([u64, u32, colp, vp], i32),
+            "am_stream_copy": ([vp, vp, u64, vp], i32),
             "am_convert": ([colp, colp, vp], i32),
             "am_filter": ([colp, am_predicate, vp, vp], i32),
             "am_compressstore": ([colp, vp, predp, colp, vp, vp, vp, u64, vp], i32),
@@ -251,6 +252,15 @@ def gen_mask_bernoulli(seed, n, selectivity, stream=None):
     _check(lib().am_gen_mask_bernoulli(seed, selectivity, ctypes.c_void_p(m.data_ptr()), n, _stream(stream)),
            "am_gen_mask_bernoulli")
     return m
+
+
+def stream_copy(src, dst, stream=None):
+    """Copy device int64 tensor `src` into `dst` (same length, 16-byte aligned)
+    with non-temporal 16-byte loads and stores (the sustained copy rate)."""
+    assert src.numel() == dst.numel()
+    _check(lib().am_stream_copy(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), src.numel(),
+                                _stream(stream)), "am_stream_copy")
+    return dst
 
 
 def gen_mask_runs(seed, n, selectivity, mean_run=256.0, stream=None):

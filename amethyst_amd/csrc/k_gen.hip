@@ -358,3 +358,44 @@ extern "C" int am_gen_mask_runs(uint64_t seed, double selectivity, double mean_r
                        splitmix_seed(seed), thr(p10), thr(p01), thr(selectivity), mask, n, chunk_words);
     return check_launch();
 }
+
+// ------------------------------------------------------------ stream copy
+//
+// Measurement harness: a plain device copy with 16-byte non-temporal loads
+// and stores, 4 per lane in flight -- the sustained read+write rate that the
+// operators' HBM fractions are compared against.
+__global__ __launch_bounds__(256) void k_stream_copy(const u64x2* src, u64x2* dst, uint64_t n2)
+{
+    constexpr uint32_t U = 4;
+    const uint64_t stride = (uint64_t)gridDim.x * 256 * U;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 * U + threadIdx.x; i < n2; i += stride) {
+        u64x2 v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t j = i + u * 256;
+            v[u] = j < n2 ? __builtin_nontemporal_load(src + j) : u64x2{0, 0};
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint64_t j = i + u * 256;
+            if (j < n2) st_nt(dst + j, v[u]);
+        }
+    }
+}
+
+extern "C" int am_stream_copy(const uint64_t* src, uint64_t* dst, uint64_t nwords, void* stream)
+{
+    if (nwords && (!src || !dst)) return AM_E_INVALID;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15) != 0) return AM_E_INVALID; // 16-byte aligned buffers
+    if (nwords == 0) return AM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t n2 = nwords / 2;
+    if (n2) {
+        const uint64_t blocks = (n2 + 1023) / 1024;
+        hipLaunchKernelGGL(k_stream_copy, dim3((uint32_t)(blocks < 16384 ? blocks : 16384)), dim3(256), 0, s,
+                           (const u64x2*)src, (u64x2*)dst, n2);
+    }
+    if (nwords & 1 && hipMemcpyAsync(dst + nwords - 1, src + nwords - 1, 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return AM_E_HIP;
+    return check_launch();
+}

@@ -186,6 +186,13 @@ int am_gen_mask_bernoulli(uint64_t seed, double selectivity, uint64_t* mask, uin
  * Counterpart of create_bitmask's clustered masks (src/main.cu:59-108). */
 int am_gen_mask_runs(uint64_t seed, double selectivity, double mean_run, uint64_t* mask, uint64_t n, void* stream);
 
+/* Copy nwords 64-bit words (16-byte aligned buffers) with 16-byte
+ * non-temporal loads and stores: the device's sustained read+write rate,
+ * against which the operators' HBM fractions are read. No reference
+ * counterpart: measurement harness, beside the event timing of
+ * CUDA_QUICKTIME (src/cuda_time.cuh:43-58). */
+int am_stream_copy(const uint64_t* src, uint64_t* dst, uint64_t nwords, void* stream);
+
 /* ---------------------------------------------------------------- operators */
 
 /* Convert a column between formats/widths: out[i] = in[i] & mask(out.bits).

@@ -46,6 +46,18 @@ def test_prng_matches_reference_stream(am, oracle_mod):
             assert np.array_equal(got, exp)
 
 
+def test_stream_copy(am, oracle_mod):
+    """The harness copy moves every word (odd lengths: the tail word too)."""
+    import torch
+    for n in (1, 2, 3, 1001, 4096, (1 << 20) + 1):
+        say("stream_copy", n)
+        src = torch.randint(-2**62, 2**62, (n,), dtype=torch.int64, device="cuda")
+        dst = torch.zeros_like(src)
+        am.stream_copy(src, dst)
+        torch.cuda.synchronize()
+        assert torch.equal(src, dst), n
+
+
 def test_gen_uniform(am, oracle_mod):
     for fmt in FORMATS:
         for bits in (3, 11, 32, 40, 64):
