@@ -282,6 +282,73 @@ __global__ __launch_bounds__(256) void k_sum_swar(SumSwar g, uint64_t* partial)
     if (threadIdx.x == 0) partial[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 }
 
+// Bitstream sum, SWAR form (2 <= w <= 32, w not dividing 64): a lane takes
+// word q and the fields that START in it, at offsets off, off+w, ... The
+// 128-bit pair (word q+1 : word q) shifted right by `off` holds the first
+// F = floor(64/w) of them at bits 0, w, 2w, ... -- a slab word of F fields,
+// summed by the same even/odd slot accumulation as k_sum_swar -- and at most
+// one more field starts below bit 64, taken alone. The last word (fields
+// past n) takes the per-field path. About 5 lane operations per element
+// instead of about 10: the kernel is bound by integer issue, not bytes.
+__global__ __launch_bounds__(256) void k_sum_bits_swar(SumSwar g, uint64_t nbits, uint64_t* partial)
+{
+    __shared__ uint64_t s_part[4];
+    const uint32_t w = g.fw, F = g.eps, nslots = (F + 1) / 2;
+    const uint64_t vmask = lowmask(w);
+    const uint32_t vm = (uint32_t)vmask;
+    const uint64_t wmagic = (1ull << 32) / w + 1; // x / w exact below 2^24
+    uint64_t total = 0, ae = 0, ao = 0;
+    uint32_t cnt = 0;
+    const uint64_t gstride = (uint64_t)gridDim.x * 256 * SUM_UNROLL;
+    for (uint64_t qb = (uint64_t)blockIdx.x * 256 * SUM_UNROLL; qb < g.words; qb += gstride) {
+        const uint32_t r0 = (uint32_t)((qb * 64) % w); // bits of an element before word qb's bit 0, mod w
+        uint64_t wv[SUM_UNROLL], nv[SUM_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+            const uint64_t q = qb + u * 256 + threadIdx.x;
+            wv[u] = q < g.words ? g.p[q] : 0;
+            nv[u] = q + 1 < g.words ? g.p[q + 1] : 0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+            const uint32_t d = u * 256 + threadIdx.x;
+            const uint64_t q = qb + d;
+            if (q >= g.words) continue;
+            const uint32_t xo = r0 + d * 64; // < 2^17
+            const uint32_t rx = xo - (uint32_t)(((uint64_t)xo * wmagic) >> 32) * w;
+            const uint32_t off = rx ? w - rx : 0; // first element start in the word
+            const uint32_t l0 = (uint32_t)wv[u], h0 = (uint32_t)(wv[u] >> 32), l1 = (uint32_t)nv[u];
+            if (q + 1 < g.words) {
+                const uint64_t x = off ? (wv[u] >> off) | (nv[u] << (64 - off)) : wv[u];
+                ae += x & g.E;
+                ao += (x >> w) & g.O;
+                if (g.has_top) total += (x >> g.top_sh) & vmask;
+                const uint32_t pe = off + F * w; // the one more field, if it starts in the word (pe >= 33)
+                if (pe < 64) total += __builtin_amdgcn_alignbit(l1, h0, pe - 32) & vm;
+            }
+            else { // the last word: only fields starting below n*w
+                const uint64_t left = nbits - q * 64;
+                const uint32_t lim = left < 64 ? (uint32_t)left : 64;
+                for (uint32_t o = off; o < lim; o += w) {
+                    const uint32_t hi = o < 32 ? h0 : l1, lo = o < 32 ? l0 : h0;
+                    total += __builtin_amdgcn_alignbit(hi, lo, o & 31) & vm;
+                }
+            }
+        }
+        cnt += SUM_UNROLL;
+        if (cnt >= g.K) {
+            total += swar_slots(ae, w, nslots) + swar_slots(ao, w, nslots);
+            ae = ao = 0;
+            cnt = 0;
+        }
+    }
+    total += swar_slots(ae, w, nslots) + swar_slots(ao, w, nslots);
+    total = wave_sum_u64(total);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+}
+
 template <int IK, bool WIDE>
 __global__ __launch_bounds__(256) void k_sum_partial(Col in, uint64_t* partial)
 {
@@ -475,6 +542,29 @@ extern "C" int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t
         if (sb > 2048) sb = 2048;
         hipLaunchKernelGGL(k_sum_swar, dim3((uint32_t)sb), b, 0, s, sw, partial);
         hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partial, (uint32_t)sb, sum_dev);
+        return check_launch();
+    }
+    if (c.kind == K_BITSTREAM && c.bits >= 2 && c.bits <= 32 && c.n) {
+        // the SWAR form over fields at a stride of `bits` (F = floor(64/bits) per word)
+        SumSwar sw{};
+        sw.p = c.p;
+        sw.fw = c.bits;
+        sw.eps = 64 / c.bits;
+        sw.bits = c.bits;
+        sw.words = (c.n * c.bits + 63) / 64;
+        const uint64_t vm = lowmask(c.bits);
+        const uint32_t ne = sw.eps & 1 ? sw.eps - 1 : sw.eps;
+        for (uint32_t k = 0; k < ne; k += 2) {
+            sw.E |= vm << (k * c.bits);
+            sw.O |= vm << (k * c.bits);
+        }
+        sw.has_top = sw.eps & 1;
+        sw.top_sh = (sw.eps - 1) * c.bits;
+        // a 2w-bit slot holds K values < 2^w while K*(2^w - 1) < 2^(2w)
+        sw.K = c.bits >= 16 ? (1u << 16) : (1u << c.bits);
+        if (sw.K < SUM_UNROLL) sw.K = SUM_UNROLL;
+        hipLaunchKernelGGL(k_sum_bits_swar, g, b, 0, s, sw, c.n * c.bits, partial);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partial, (uint32_t)blocks, sum_dev);
         return check_launch();
     }
     switch (c.kind) {

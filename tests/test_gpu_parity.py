@@ -318,6 +318,18 @@ def test_sum_swar_saturated(am, oracle_mod):
         col = am.Column.from_host_words(np.full(words, word, dtype=np.uint64), n, bits, 1)
         got = int(am.u64(am.sum(col, ws=ws))[0])
         assert got == n * ((1 << bits) - 1), (bits, n, got)
+    # bitstreams (SWAR over fields at a stride of w): all-ones streams, long
+    # enough that each lane reaches its flush bound (K = 2^w words)
+    for bits, n in ((3, 3 * (1 << 24) + 7), (5, (1 << 25) + 3), (11, (1 << 26) + 1)):
+        words = (n * bits + 63) // 64
+        w = np.full(words, np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+        tail = n * bits - 64 * (words - 1)
+        if tail < 64:
+            w[-1] = np.uint64((1 << tail) - 1)
+        say("sum-saturated bitstream", bits, n)
+        col = am.Column.from_host_words(w, n, bits, 2)
+        got = int(am.u64(am.sum(col, ws=ws))[0])
+        assert got == n * ((1 << bits) - 1), (bits, n, got)
 
 
 def _width(v):
