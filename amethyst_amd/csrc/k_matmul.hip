@@ -346,9 +346,9 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
                     const int32_t* src = slot(row >> 6) + (row & 63) * MM_C_STRIDE + (t & 63) * 4;
                     int32_t* dst = J.Ci + (uint64_t)gr * J.N + c;
                     if (c + 4 <= J.N && ((uintptr_t)dst & 15) == 0) {
-                        *(v4i*)dst = *(const v4i*)src;
+                        st_nt((v4i*)dst, *(const v4i*)src);
                     } else {
-                        for (uint32_t q = 0; q < 4 && c + q < J.N; q++) dst[q] = src[q];
+                        for (uint32_t q = 0; q < 4 && c + q < J.N; q++) st_nt(dst + q, src[q]);
                     }
                 }
             }
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
                     uint64_t word = 0;
                     for (uint32_t q = 0; q < eps_o && c0 + q < J.N; q++)
                         word |= ((uint64_t)(uint32_t)src[q] & omask) << (q * ob);
-                    J.Cp[(uint64_t)gr * rw_o + c0 / eps_o] = word;
+                    st_nt(J.Cp + (uint64_t)gr * rw_o + c0 / eps_o, word);
                 }
             }
             __syncthreads(); // the half's rows are stored before the next half lands
