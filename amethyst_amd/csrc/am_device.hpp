@@ -740,4 +740,85 @@ struct Lookback {
     }
 };
 
+
+// Write the output words a tile of compacted values completes: ctile values
+// s_vals[0, ctile) (LDS) at element P of a packed column `o` (slab or
+// bitstream, capacity ocap elements; values past it stay unwritten). Values
+// before P come from `carry`, the 63 output-stream bits before the tile
+// (bit 62 = the bit just before P's first bit). The last tile also writes
+// its final partial word. Tile-relative indices fit 32 bits (a tile holds
+// < 2^17 values); nthr threads, this one tid.
+template <int OK, typename V>
+__device__ inline void write_packed_tile(const Col& o, uint64_t ocap, const V* s_vals, uint64_t P, uint64_t carry,
+                                         uint32_t ctile, bool last, uint32_t tid, uint32_t nthr)
+{
+    const uint32_t bo = o.bits;
+    const uint64_t om = o.mask;
+    if (OK == K_SLAB) {
+        // word kf + j holds tile values [j*eps - q0, (j+1)*eps - q0); values
+        // before the tile come from the carried window. Tile-relative
+        // indices fit 32 bits (a tile holds < 2^17 values).
+        const uint32_t eps = o.eps;
+        const uint64_t kf = P / eps;
+        const int32_t q0 = (int32_t)(P - kf * eps); // carried values in the first word
+        uint64_t kl = (P + ctile) / eps;
+        if (last && ((P + ctile) % eps)) kl += 1;
+        const uint64_t wcap = (ocap + eps - 1) / eps;
+        if (kl > wcap) kl = wcap;
+        const uint32_t nk = kl > kf ? (uint32_t)(kl - kf) : 0;
+        // values at or past min(ctile, ocap - P) stay zero
+        const int64_t room = (int64_t)ocap - (int64_t)P;
+        const int32_t lim = room < (int64_t)ctile ? (int32_t)(room < -(int64_t)eps ? -(int64_t)eps : room) : (int32_t)ctile;
+        for (uint32_t j = tid; j < nk; j += nthr) {
+            const int32_t i0 = (int32_t)(j * eps) - q0;
+            uint64_t word = 0;
+            if (i0 >= 0 && i0 + (int32_t)eps <= lim) { // whole word from the staged values
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += bo) word |= (uint64_t)s_vals[i0 + s] << sh;
+            }
+            else {
+                for (uint32_t s = 0, sh = 0; s < eps; s++, sh += bo) {
+                    const int32_t i = i0 + (int32_t)s;
+                    if (i >= lim) break;
+                    const uint64_t v = i < 0 ? (carry >> (63 - (uint32_t)(-i) * bo)) & om : (uint64_t)s_vals[i];
+                    word |= v << sh;
+                }
+            }
+            st_nt(o.p + kf + j, word);
+        }
+    }
+    else {
+        // word kf + j covers tile stream bits [j*64 - o0, j*64 - o0 + 64);
+        // bits before the tile come from the carried window
+        const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
+        const uint64_t kf = B0 >> 6;
+        const int32_t o0 = (int32_t)(B0 & 63);
+        uint64_t kl = B1 >> 6;
+        if (last && (B1 & 63)) kl += 1;
+        const uint64_t wcap = (Bcap + 63) >> 6;
+        if (kl > wcap) kl = wcap;
+        const uint32_t nk = kl > kf ? (uint32_t)(kl - kf) : 0;
+        // x / bo = (x * (floor(2^32/bo) + 1)) >> 32, exact for x < 2^24 (the
+        // multiplier needs 33 bits when bo = 1)
+        const uint64_t bmagic = (1ull << 32) / bo + 1;
+        auto div_bo = [&](uint32_t x) { return (uint32_t)(((uint64_t)x * bmagic) >> 32); };
+        for (uint32_t j = tid; j < nk; j += nthr) {
+            const int32_t rb = (int32_t)(j * 64) - o0; // tile stream bit of the word's bit 0
+            uint64_t word = 0;
+            uint32_t i = 0;
+            if (rb < 0) word = carry >> (63 + rb); // carry bit 62 = stream bit B0 - 1
+            else i = div_bo((uint32_t)rb);
+            uint32_t ie = div_bo((uint32_t)(rb + 63 + (int32_t)bo)); // one past the last value touching the word
+            if (ie > ctile) ie = ctile;
+            for (; i < ie; i++) {
+                const int32_t pos = (int32_t)(i * bo) - rb;
+                const uint64_t v = s_vals[i];
+                word |= pos >= 0 ? (v << pos) : (v >> (-pos));
+            }
+            const uint64_t b0 = (kf + j) * 64;
+            if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
+            st_nt(o.p + kf + j, word);
+        }
+    }
+}
+
 } // namespace am

@@ -188,52 +188,8 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
     const uint64_t ocap = J.s.n;
 
     // ---- 3. write the words this tile completes
-    if (J.s.kind == K_SLAB) {
-        const uint64_t eps = J.s.eps;
-        const uint64_t kf = P / eps;
-        uint64_t kl = (P + ctile) / eps;
-        if (last && ((P + ctile) % eps)) kl += 1;
-        const uint64_t wcap = (ocap + eps - 1) / eps;
-        if (kl > wcap) kl = wcap;
-        for (uint64_t k = kf + tid; k < kl; k += 256) {
-            uint64_t word = 0;
-            const uint64_t e0 = k * eps;
-            for (uint32_t s = 0; s < eps; s++) {
-                const uint64_t e = e0 + s;
-                if (e >= ocap) break;
-                uint64_t v;
-                if (e < P) v = (carry >> (63 - (P - e) * bo)) & om;
-                else if (e < P + ctile) v = s_vals[e - P];
-                else break;
-                word |= v << (s * bo);
-            }
-            J.s.p[k] = word;
-        }
-    }
-    else {
-        const uint64_t B0 = P * bo, B1 = (P + ctile) * bo, Bcap = ocap * bo;
-        const uint64_t kf = B0 >> 6;
-        uint64_t kl = B1 >> 6;
-        if (last && (B1 & 63)) kl += 1;
-        const uint64_t wcap = (Bcap + 63) >> 6;
-        if (kl > wcap) kl = wcap;
-        for (uint64_t k = kf + tid; k < kl; k += 256) {
-            const uint64_t b0 = k * 64;
-            uint64_t word = 0;
-            if (b0 < B0) word = carry >> (b0 + 63 - B0);
-            uint64_t e0 = b0 / bo;
-            if (e0 < P) e0 = P;
-            uint64_t e1 = (b0 + 63) / bo + 1;
-            if (e1 > P + ctile) e1 = P + ctile;
-            for (uint64_t e = e0; e < e1; e++) {
-                const int64_t pos = (int64_t)(e * bo) - (int64_t)b0;
-                const uint64_t v = s_vals[e - P];
-                word |= pos >= 0 ? (v << pos) : (v >> (-pos));
-            }
-            if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-            J.s.p[k] = word;
-        }
-    }
+    if (J.s.kind == K_SLAB) write_packed_tile<K_SLAB>(J.s, ocap, s_vals, P, carry, ctile, last, tid, 256);
+    else write_packed_tile<K_BITSTREAM>(J.s, ocap, s_vals, P, carry, ctile, last, tid, 256);
 }
 
 __global__ void k_epoch_bump(WsHeader* hdr)
