@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
     __shared__ uint64_t s_wsum[PL_WAVES];
     __shared__ uint64_t s_P, s_carry;
     __shared__ uint32_t s_epoch, s_abort;
-    __shared__ uint64_t s_vals[pl_tile(PL_J)];
+    __shared__ uint32_t s_vals[pl_tile(PL_J)]; // a+b of <= 31-bit inputs fits 32 bits
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t g = blockIdx.x; // global tile (one per block, in order)
@@ -109,17 +109,17 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
     }
     uint64_t mine = 0, vsum = 0;
     uint32_t cnt = 0, ovf = 0;
-    uint64_t val[PL_J];
+    uint32_t val[PL_J];
 #pragma unroll
     for (uint32_t j = 0; j < PL_J; j++) {
         const uint64_t x = wrem ? fa[j].value(ra, idx(j)) : 0, y = wrem ? fb[j].value(rb, idx(j)) : 0;
         const bool fl = (j * 64 + lane < wrem) && J.pr.sel32((uint32_t)x);
-        val[j] = (x + y) & vm;
+        val[j] = (uint32_t)((x + y) & vm);
         const uint64_t bl = ballot(fl);
         if (lane == j) mine = bl;
         cnt += (uint32_t)__popcll(bl);
         vsum += fl ? val[j] : 0;
-        if (J.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && (val[j] & ~om) != 0));
+        if (J.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && ((uint64_t)val[j] & ~om) != 0));
     }
     vsum = wave_sum_u64(vsum);
     if (lane == 0) {
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
         for (uint32_t j = 0; j < PL_J; j++) {
             const uint64_t bl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
                                 (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
-            if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = val[j] & om;
+            if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = (uint32_t)(val[j] & om);
             run += (uint32_t)__popcll(bl);
         }
     }
