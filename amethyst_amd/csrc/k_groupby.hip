@@ -55,6 +55,8 @@ __global__ __launch_bounds__(TH) void k_groupby_partial(GroupArgs a)
     constexpr uint32_t TILE = GB_ROWS * TH; // rows per step
     extern __shared__ uint64_t lds[];
     const uint32_t G = 1u << a.kbits;
+    if (blockIdx.x == 0) // the reduce accumulates into the totals
+        for (uint32_t c = threadIdx.x; c < 2 * G; c += TH) a.total[c] = 0;
     uint64_t* packed = lds;       // [G]
     uint64_t* wsum = lds + G;     // [G]
     uint64_t* wcnt = lds + 2 * G; // [G]
@@ -125,6 +127,8 @@ __global__ __launch_bounds__(GB_THREADS) void k_groupby_words(GroupArgs a)
 {
     extern __shared__ uint64_t lds[];
     const uint32_t G = 1u << a.kbits;
+    if (blockIdx.x == 0) // the reduce accumulates into the totals
+        for (uint32_t c = threadIdx.x; c < 2 * G; c += GB_THREADS) a.total[c] = 0;
     uint64_t* packed = lds;
     uint64_t* wsum = lds + G;
     uint64_t* wcnt = lds + 2 * G;
@@ -208,18 +212,27 @@ __global__ __launch_bounds__(GB_THREADS) void k_groupby_words(GroupArgs a)
 
 // total[c] = sum over blocks of part[b][c], c < 2G; 64 columns per block,
 // each wave sums every 4th row (coalesced 512-byte rows).
+// Column sums of the partial tables: block (x, y) sums partial rows
+// [y*R, (y+1)*R) of columns [64x, 64x+64) and adds them into the totals
+// (zeroed by the partial kernel) with 64-bit integer atomics -- exact, and
+// independent of the order the blocks finish in.
+constexpr uint32_t GB_RED_ROWS = 16; // partial rows per reduce block
 __global__ __launch_bounds__(256) void k_groupby_reduce(GroupArgs a)
 {
     __shared__ uint64_t s_acc[4][64];
     const uint32_t G2 = 2u << a.kbits;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t c = blockIdx.x * 64 + lane;
+    const uint32_t b0 = blockIdx.y * GB_RED_ROWS, b1 = min(b0 + GB_RED_ROWS, a.blocks);
     uint64_t acc = 0;
     if (c < G2)
-        for (uint32_t b = wave; b < a.blocks; b += 4) acc += a.part[(uint64_t)b * G2 + c];
+        for (uint32_t b = b0 + wave; b < b1; b += 4) acc += a.part[(uint64_t)b * G2 + c];
     s_acc[wave][lane] = acc;
     __syncthreads();
-    if (wave == 0 && c < G2) a.total[c] = s_acc[0][lane] + s_acc[1][lane] + s_acc[2][lane] + s_acc[3][lane];
+    if (wave == 0 && c < G2) {
+        const uint64_t sum = s_acc[0][lane] + s_acc[1][lane] + s_acc[2][lane] + s_acc[3][lane];
+        if (sum) atomicAdd((unsigned long long*)&a.total[c], (unsigned long long)sum);
+    }
 }
 
 __global__ __launch_bounds__(GB_THREADS) void k_groupby_writeout(GroupArgs a)
@@ -372,7 +385,8 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
         a.blocks = (uint32_t)wb;
         if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_groupby_words, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_groupby_words, dim3((uint32_t)wb), dim3(GB_THREADS), lds, s, a);
-        hipLaunchKernelGGL(k_groupby_reduce, dim3((2 * G + 63) / 64), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_groupby_reduce, dim3((2 * G + 63) / 64, (a.blocks + GB_RED_ROWS - 1) / GB_RED_ROWS), dim3(256),
+                           0, s, a);
         hipLaunchKernelGGL(k_groupby_writeout, dim3(1), dim3(GB_THREADS), 0, s, a);
         return check_launch();
     }
@@ -401,7 +415,8 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
 #undef GB_V
 #undef GB_LAUNCH
 #undef GB_GO
-    hipLaunchKernelGGL(k_groupby_reduce, dim3((2 * G + 63) / 64), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_groupby_reduce, dim3((2 * G + 63) / 64, (a.blocks + GB_RED_ROWS - 1) / GB_RED_ROWS), dim3(256), 0,
+                       s, a);
     hipLaunchKernelGGL(k_groupby_writeout, dim3(1), dim3(GB_THREADS), 0, s, a);
     return check_launch();
 }
