@@ -188,11 +188,21 @@ __device__ inline uint32_t select_fields(uint64_t x, uint32_t eps, uint32_t fw, 
 {
     const uint32_t LO = tp.lo, RG = tp.rg;
     uint32_t gr = 0; // bits in reverse field order
-    for (uint32_t s = 0, sh = 64 - bw; s < eps; s++, sh -= fw) {
-        const uint32_t t = (uint32_t)((x << sh) >> 32) - LO;
-        uint32_t ng;
-        asm("v_cmp_ge_u32 vcc, %2, %1\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc" : "=v"(ng) : "v"(t), "s"(RG), "v"(gr) : "vcc");
-        gr = ng;
+    if (LO == 0) { // (wave-uniform) ranges from 0, e.g. v < c: no subtraction
+        for (uint32_t s = 0, sh = 64 - bw; s < eps; s++, sh -= fw) {
+            const uint32_t t = (uint32_t)((x << sh) >> 32);
+            uint32_t ng;
+            asm("v_cmp_ge_u32 vcc, %2, %1\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc" : "=v"(ng) : "v"(t), "s"(RG), "v"(gr) : "vcc");
+            gr = ng;
+        }
+    }
+    else {
+        for (uint32_t s = 0, sh = 64 - bw; s < eps; s++, sh -= fw) {
+            const uint32_t t = (uint32_t)((x << sh) >> 32) - LO;
+            uint32_t ng;
+            asm("v_cmp_ge_u32 vcc, %2, %1\n\tv_addc_co_u32 %0, vcc, %3, %3, vcc" : "=v"(ng) : "v"(t), "s"(RG), "v"(gr) : "vcc");
+            gr = ng;
+        }
     }
     return (__builtin_bitreverse32(gr) >> (32 - eps)) ^ tp.inv;
 }
@@ -200,6 +210,19 @@ __device__ inline uint32_t select_fields(uint64_t x, uint32_t eps, uint32_t fw, 
 // ------------------------------------------------------------ wave64 helpers
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Workgroup barrier over LDS only: waits for this wave's LDS operations
+// (lgkmcnt(0)), then s_barrier. __syncthreads() also drains the wave's
+// global stores (its fence waits vmcnt(0)), a full memory round trip that
+// nothing after these barriers needs: no wave reads global memory that
+// another wave of the block wrote before them.
+__device__ inline void lds_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
 
 // number of set bits of m below this lane (v_mbcnt_lo/hi)
 __device__ inline uint32_t mbcnt(uint64_t m)

@@ -209,10 +209,11 @@ constexpr uint32_t CD_THREADS = CD_WAVES * 64;
 constexpr uint32_t CD_STAGE = 19456; // LDS bytes: input words, then compacted values (8 blocks per CU)
 constexpr uint32_t CD_STAGE_C = 16384; // container outputs: the same, compacted values in passes (8 blocks per CU)
 
-// Copy values s_vals[0, cnt) to out[P, P + cnt) (capacity-checked), four
-// per thread step with one vector store where the output is aligned.
+// Copy values s_vals[0, cnt) >> k to out[P, P + cnt) (capacity-checked),
+// four per thread step with one vector store where the output is aligned.
+// (k drops the bits beneath a value staged top-aligned in its slot.)
 template <typename C, typename V>
-__device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t P, uint32_t cnt, uint32_t tid)
+__device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t P, uint32_t cnt, uint32_t tid, uint32_t k)
 {
     C* out = (C*)a.out.p;
     const uint64_t cap = a.cap;
@@ -220,12 +221,13 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
     if (!a.out_vec) head = cnt; // unaligned output buffer: element stores only
     if (head > cnt) head = cnt;
     for (uint32_t i = tid; i < head; i += CD_THREADS)
-        if (P + i < cap) st_nt(out + P + i, (C)s_vals[i]);
+        if (P + i < cap) st_nt(out + P + i, (C)(s_vals[i] >> k));
     const uint32_t nq = (cnt - head) / 4;
     for (uint32_t q = tid; q < nq; q += CD_THREADS) {
         const uint32_t i0 = head + 4 * q;
         const uint64_t r = P + i0;
-        const C v0 = (C)s_vals[i0], v1 = (C)s_vals[i0 + 1], v2 = (C)s_vals[i0 + 2], v3 = (C)s_vals[i0 + 3];
+        const C v0 = (C)(s_vals[i0] >> k), v1 = (C)(s_vals[i0 + 1] >> k), v2 = (C)(s_vals[i0 + 2] >> k),
+                v3 = (C)(s_vals[i0 + 3] >> k);
         if (r + 4 <= cap) {
             if constexpr (sizeof(C) == 4) st_nt((u32x4*)(out + r), u32x4{(uint32_t)v0, (uint32_t)v1, (uint32_t)v2, (uint32_t)v3});
             else if constexpr (sizeof(C) == 2)
@@ -245,7 +247,7 @@ __device__ inline void copy_out(const CompressArgs& a, const V* s_vals, uint64_t
         }
     }
     for (uint32_t i = head + 4 * nq + tid; i < cnt; i += CD_THREADS)
-        if (P + i < cap) st_nt(out + P + i, (C)s_vals[i]);
+        if (P + i < cap) st_nt(out + P + i, (C)(s_vals[i] >> k));
 }
 
 // container outputs of a count-only scan and packed outputs fit 8 waves per
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         s_epoch = ep;
         s_abort = ab;
     }
-    __syncthreads(); // every word is in registers: s_buf is free for the compacted values
+    lds_barrier(); // every word is in registers: s_buf is free for the compacted values
     if (s_abort) {
         if (last && tid == 0 && a.stats) a.stats[0] = STATS_INVALID;
         return;
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
             wp = wave_or_u64(wp);
         }
         if (lane == 0) s_win[wave] = wp;
-        __syncthreads();
+        lds_barrier();
         own_w = (s_win[0] | s_win[1] | s_win[2] | s_win[3]) & WIN_BITS;
     }
     if (lead) {
@@ -474,6 +476,21 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
     // more), packed outputs always fit
     constexpr uint32_t CAP = STAGE / sizeof(V);
     const uint32_t vm32 = (uint32_t)(a.in.mask & a.out.mask);
+    // Container outputs stage a field's low vbits bits top-aligned in its
+    // slot (one shift of the word, no mask; the bits beneath are neighbouring
+    // fields', dropped by copy_out's shift by ksh) at a running byte offset.
+    constexpr bool TOP = OK == K_STRIDED && !WIDE;
+    const uint32_t vbits = bw < a.out.bits ? bw : a.out.bits;
+    const uint32_t ksh = TOP ? 8 * (uint32_t)sizeof(V) - vbits : 0;
+    auto stage_top = [&](uint64_t xr, uint32_t gg, uint32_t d, bool chk) {
+        uint32_t da = d * (uint32_t)sizeof(V);
+        for (uint32_t s = 0, sh = 64 - vbits; s < eps; s++, sh -= fw)
+            if (__builtin_amdgcn_ubfe(gg, s, 1)) {
+                if (!chk || da < CAP * (uint32_t)sizeof(V))
+                    *(V*)((uint8_t*)s_vals + da) = (V)((xr << sh) >> (64 - 8 * sizeof(V)));
+                da += (uint32_t)sizeof(V);
+            }
+    };
     auto stage = [&](uint32_t base, auto checked) {
         constexpr bool CHK = decltype(checked)::value;
 #pragma unroll
@@ -485,6 +502,8 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
             if (WIDE) {
                 if (!CHK || d < CAP) s_vals[d] = (V)(x[r] & a.in.mask & a.out.mask);
             }
+            else if (TOP)
+                stage_top(x[r], gg, d, CHK);
             else // (a branch-free form storing every field, unselected ones to a
                  // trash slot, measured slower: the extra LDS stores cost more)
                 for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
@@ -502,19 +521,19 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
             L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
             tile_lookback<F, true>(a, L, t, ctile, otile, 0, lane, &s_P, nullptr);
         }
-        __syncthreads();
+        lds_barrier();
         const uint64_t P = s_P;
         for (uint32_t base = 0;;) {
             const uint32_t cnt = ctile - base < CAP ? ctile - base : CAP;
             switch (a.out.cbytes) {
-                case 1: copy_out<uint8_t>(a, s_vals, P + base, cnt, tid); break;
-                case 2: copy_out<uint16_t>(a, s_vals, P + base, cnt, tid); break;
-                case 4: copy_out<uint32_t>(a, s_vals, P + base, cnt, tid); break;
-                default: copy_out<uint64_t>(a, s_vals, P + base, cnt, tid);
+                case 1: copy_out<uint8_t>(a, s_vals, P + base, cnt, tid, ksh); break;
+                case 2: copy_out<uint16_t>(a, s_vals, P + base, cnt, tid, ksh); break;
+                case 4: copy_out<uint32_t>(a, s_vals, P + base, cnt, tid, ksh); break;
+                default: copy_out<uint64_t>(a, s_vals, P + base, cnt, tid, ksh);
             }
             base += CAP;
             if (base >= ctile) break;
-            __syncthreads(); // the pass is out: stage the next
+            lds_barrier(); // the pass is out: stage the next
             // (chunk by chunk from the tile's words again: the words, bits
             // and ranks are not kept live across the look-back, which would
             // cost occupancy)
@@ -528,6 +547,8 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
                 if (WIDE) {
                     if (gg && d < CAP) s_vals[d] = (V)(xr & a.in.mask & a.out.mask);
                 }
+                else if (TOP)
+                    stage_top(xr, gg, d, true);
                 else
                     for (uint32_t s = 0, sh = 0; s < eps; s++, sh += fw)
                         if (__builtin_amdgcn_ubfe(gg, s, 1)) {
@@ -535,7 +556,7 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
                             d++;
                         }
             }
-            __syncthreads();
+            lds_barrier();
         }
         if (last && tid == 0) {
             uint64_t K = P + ctile;
@@ -547,13 +568,13 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
     else {
         // packed outputs: output words are assembled from the staged values
         // and the carried window of the predecessors
-        __syncthreads();
+        lds_barrier();
         if (lead) {
             const Lookback<F> L = make_lookback<F>(a, s_epoch);
             L.clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
             tile_lookback<F, true>(a, L, t, ctile, otile, own_w, lane, &s_P, &s_carry);
         }
-        __syncthreads();
+        lds_barrier();
         tile_write<OK>(a, s_vals, s_P, s_carry, ctile, last, tid);
     }
 }
