@@ -645,7 +645,7 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
         s_wcnt[wave] = cnt;
         s_wovf[wave] = ovf;
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t wpre = 0, ctile = 0, otile = 0;
 #pragma unroll
     for (uint32_t w = 0; w < 4; w++) {
@@ -662,13 +662,13 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
         if ((b >> lane) & 1) s_vals[run + mbcnt(b)] = f[j].value(rd, idx(j)) & om;
         run += (uint32_t)__popcll(b);
     }
-    __syncthreads();
+    lds_barrier();
     if (OK != K_STRIDED) {
         if (wave == 0) {
             const uint64_t w = tile_window(s_vals, ctile, a.out.bits, lane);
             tile_lookback<F>(a, L, t, ctile, otile, w, lane, &s_P, &s_carry);
         }
-        __syncthreads();
+        lds_barrier();
     }
     tile_write<OK>(a, s_vals, s_P, s_carry, ctile, t == a.T - 1, tid);
 }

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
             }
         }
     }
-    __syncthreads();
+    lds_barrier(); // (wave 0's descriptor and stats stores need not drain)
     const uint64_t P = s_P, carry = s_carry;
     const bool last = (t == J.T - 1);
     const uint64_t ocap = J.s.n;
