@@ -83,38 +83,64 @@ __device__ inline uint32_t find_job(const MatJob* jobs, uint32_t njobs, uint32_t
 // A -> int8 rows [M][Kp]: one item = one row and 16 consecutive k (one
 // 16-byte store). The 16 fields span at most 3 packed words (>= 9 fields per
 // word); neighbouring lanes take neighbouring items of a row, so the word
-// loads of a wave cover a contiguous stretch of the row.
+// loads of a wave cover a contiguous stretch of the row. A thread issues the
+// loads of all its items before unpacking any (the int8 stores may alias the
+// packed input as far as the compiler knows, so it would not hoist a later
+// item's loads above an earlier item's store by itself).
 template <uint32_t BITS>
 __global__ __launch_bounds__(256) void k_mm_unpack_a(const MatJob* jobs, const MatSeg* segs)
 {
-    constexpr uint32_t EPS = 64 / BITS;
+    constexpr uint32_t EPS = 64 / BITS, Q = MU_A_ITEMS / 256;
     constexpr uint64_t MASK = (1ull << BITS) - 1;
     const MatSeg sg = segs[blockIdx.x];
     const MatJob& J = jobs[sg.job];
     const uint32_t cpr = J.Kp / 16, rw = (J.K + EPS - 1) / EPS;
     const uint32_t items = J.M * cpr; // < 2^32 (host-checked)
+    uint64_t wv[Q][3];
 #pragma unroll
-    for (uint32_t q = 0; q < MU_A_ITEMS / 256; q++) {
+    for (uint32_t q = 0; q < Q; q++) {
+        const uint32_t it = sg.first + q * 256 + threadIdx.x;
+        wv[q][0] = wv[q][1] = wv[q][2] = 0;
+        if (it >= items) continue;
+        const uint32_t r = it / cpr, k0 = (it - r * cpr) * 16;
+        if (k0 >= J.K) continue;
+        const uint64_t* row = J.A + (uint64_t)r * rw;
+        const uint32_t w = k0 / EPS;
+        wv[q][0] = row[w];
+        if (w + 1 < rw) wv[q][1] = row[w + 1];
+        if (w + 2 < rw) wv[q][2] = row[w + 2];
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
         const uint32_t it = sg.first + q * 256 + threadIdx.x;
         if (it >= items) break;
         const uint32_t r = it / cpr, k0 = (it - r * cpr) * 16;
         uint32_t v[4] = {0, 0, 0, 0};
         if (k0 < J.K) {
-            const uint64_t* row = J.A + (uint64_t)r * rw;
+            // the three words' payloads (dead top bits dropped) as one
+            // 192-bit stream s0:s1:s2, shifted down to the item's first field:
+            // field s is then bits [BITS*s, BITS*s + BITS) of t0:t1
+            constexpr uint32_t L = EPS * BITS; // payload bits per word
             const uint32_t w = k0 / EPS, off = k0 - w * EPS;
-            uint64_t cur = row[w] >> (off * BITS);
-            uint64_t nxt = w + 1 < rw ? row[w + 1] : 0, nxt2 = w + 2 < rw ? row[w + 2] : 0;
-            uint32_t left = EPS - off;
+            uint64_t s0 = wv[q][0], s1 = wv[q][1], s2 = wv[q][2];
+            if constexpr (L < 64) {
+                constexpr uint64_t PM = (1ull << L) - 1;
+                s0 = (s0 & PM) | (s1 << L);
+                s1 = ((s1 & PM) >> (64 - L)) | (s2 << (2 * L - 64));
+                s2 = (s2 & PM) >> (128 - 2 * L);
+            }
+            const uint32_t sh = off * BITS; // < 64
+            uint64_t t0 = (s0 >> sh) | ((s1 << 1) << (63 - sh));
+            uint64_t t1 = (s1 >> sh) | ((s2 << 1) << (63 - sh));
+            // fields past K (end of the row) read as 0
+            const uint32_t nb = (J.K - k0 < 16 ? J.K - k0 : 16) * BITS; // valid bits, <= 112
+            if (nb < 64) t0 &= (1ull << nb) - 1;
+            t1 &= nb <= 64 ? 0ull : (1ull << (nb - 64)) - 1;
 #pragma unroll
-            for (uint32_t s = 0; s < 16; s++) {
-                const uint32_t x = k0 + s < J.K ? (uint32_t)(cur & MASK) : 0u;
-                v[s >> 2] |= x << (8 * (s & 3));
-                cur >>= BITS;
-                if (--left == 0) {
-                    cur = nxt;
-                    nxt = nxt2;
-                    left = EPS;
-                }
+            for (uint32_t f = 0; f < 16; f++) {
+                const uint32_t b = f * BITS;
+                const uint64_t x = b + BITS <= 64 ? (t0 >> b) : (b >= 64 ? (t1 >> (b - 64)) : ((t0 >> b) | (t1 << (64 - b))));
+                v[f >> 2] |= ((uint32_t)x & (uint32_t)MASK) << (8 * (f & 3));
             }
         }
         *(uint4*)(J.A8 + (uint64_t)r * J.Kp + k0) = make_uint4(v[0], v[1], v[2], v[3]);
@@ -138,19 +164,29 @@ __global__ __launch_bounds__(256) void k_mm_unpack_bt(const MatJob* jobs, const 
     const uint32_t ntn = (J.N + NT - 1) / NT;
     const uint32_t kt = sg.first / ntn, nt = sg.first - kt * ntn;
     const uint32_t k0 = kt * MU_KT, wb = nt * WN, rw = (J.N + EPS - 1) / EPS;
-    for (uint32_t task = threadIdx.x; task < WN * (MU_KT / 4); task += 256) {
+    // every load of the thread's tasks first, then the transposes into LDS
+    constexpr uint32_t NTASK = WN * (MU_KT / 4), TPT = (NTASK + 255) / 256;
+    uint64_t x[TPT][4];
+#pragma unroll
+    for (uint32_t p = 0; p < TPT; p++) {
+        const uint32_t task = p * 256 + threadIdx.x;
         const uint32_t q = task / WN, w = task - q * WN, gw = wb + w;
-        uint64_t x[4];
 #pragma unroll
         for (uint32_t i = 0; i < 4; i++) {
             const uint32_t k = k0 + 4 * q + i;
-            x[i] = (gw < rw && k < J.K) ? J.B[(uint64_t)k * rw + gw] : 0;
+            x[p][i] = (task < NTASK && gw < rw && k < J.K) ? J.B[(uint64_t)k * rw + gw] : 0;
         }
+    }
+#pragma unroll
+    for (uint32_t p = 0; p < TPT; p++) {
+        const uint32_t task = p * 256 + threadIdx.x;
+        if (task >= NTASK) break;
+        const uint32_t q = task / WN, w = task - q * WN;
 #pragma unroll
         for (uint32_t f = 0; f < EPS; f++) {
             uint32_t d = 0;
 #pragma unroll
-            for (uint32_t i = 0; i < 4; i++) d |= (uint32_t)((x[i] >> (f * BITS)) & MASK) << (8 * i);
+            for (uint32_t i = 0; i < 4; i++) d |= (uint32_t)((x[p][i] >> (f * BITS)) & MASK) << (8 * i);
             tr[(w * EPS + f) * LS + q] = d;
         }
     }

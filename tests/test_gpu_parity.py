@@ -406,7 +406,7 @@ def test_matmul_packed(am, oracle_mod):
     import torch
     rng = np.random.default_rng(10)
     shapes = [(1, 1, 1), (16, 16, 64), (17, 33, 65), (128, 128, 64), (130, 200, 300), (257, 129, 1000)]
-    for bits in (7, 3, 1):
+    for bits in (7, 6, 5, 4, 3, 2, 1):  # 63, 60 and 64 payload bits per word
         probs, exps = [], []
         for (M, N, K) in shapes:
             A = rng.integers(0, 1 << bits, (M, K)).astype(np.uint64)
