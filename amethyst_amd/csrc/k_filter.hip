@@ -116,8 +116,10 @@ __global__ __launch_bounds__(256) void k_filter_stream(Col in, Pred pr, uint64_t
 // words: one field per word -> the chunk's ballot; a power-of-two number of
 // fields per word -> the lanes feeding one mask word are an aligned segment
 // of 64/eps lanes, combined with a DPP OR-scan and stored by its last lane;
-// other widths -> OR-ed into the tile's mask words in LDS (at most 21 lanes
-// per word) and stored coalesced. One tile per block; no block waits on another.
+// other widths -> each lane puts its word's bits in LDS and lane m of the
+// wave gathers the wave's mask words m, m+64, ... from the (about 64/eps)
+// words each spans (wave-local, so no block barrier).
+// One tile per block; no block waits on another.
 struct FilterWordsArgs {
     Col in;
     Pred pr;
@@ -128,21 +130,21 @@ struct FilterWordsArgs {
 };
 
 constexpr uint32_t FW_RMAX = 8;
-constexpr uint32_t FW_MASKW = 1024; // LDS mask words per tile (non-power-of-two eps)
+constexpr uint32_t FW_SELW = 4 * 64 * FW_RMAX; // LDS selection words (u32) per tile
 
-template <bool WIDE>
+template <bool WIDE, bool GATHER>
 __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
 {
-    __shared__ uint64_t s_mask[FW_MASKW];
+    __shared__ uint32_t s_sel[FW_SELW]; // non-power-of-two eps: selection bits per word
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t t = blockIdx.x;
     const uint32_t fw = a.fw, eps = a.eps, R = a.R, bw = a.in.bits;
-    const bool lds = !WIDE && !a.pow2;
+    constexpr bool lds = GATHER; // non-power-of-two eps (never WIDE)
     const uint64_t n = a.in.n, nw = a.in_words;
     const uint64_t w0 = t * (uint64_t)(256 * R);
     const uint64_t nmw = (n + 63) >> 6;
-    const uint32_t NMW = 4 * R * eps;
     const uint64_t mw0 = (w0 * eps) >> 6;
+    uint32_t* const sg = s_sel + wave * 64 * R; // this wave's words
     uint64_t x[FW_RMAX];
 #pragma unroll
     for (uint32_t r = 0; r < FW_RMAX; r++) {
@@ -150,12 +152,10 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
         const uint64_t wi = w0 + (uint64_t)(wave * R + r) * 64 + lane;
         if (r < R && wi < nw) x[r] = a.in.p[wi];
     }
-    if (lds) {
-        for (uint32_t i = tid; i < NMW; i += 256) s_mask[i] = 0;
-        __syncthreads();
-    }
+    // (all R loads in flight before any word is used: without this the
+    // compiler sinks each load to its chunk and the loads go out one by one)
+    asm volatile("" ::: "memory");
     const bool last = t == gridDim.x - 1;
-    const uint32_t leps = __umul24(lane, eps);
     const uint32_t S = 64 / eps; // lanes per mask word (power-of-two eps)
 #pragma unroll
     for (uint32_t r = 0; r < FW_RMAX; r++) {
@@ -177,16 +177,26 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
             const uint64_t mi = cmw + lane / S;
             if ((lane & (S - 1)) == S - 1 && mi < nmw) st_nt(a.mask_out + mi, m);
         }
-        else if (gg) {
-            const uint32_t p = ci * 64 * eps + leps, w = p >> 6, off = p & 63;
-            atomicOr((unsigned long long*)&s_mask[w], (unsigned long long)gg << off);
-            if (off && off + eps > 64) atomicOr((unsigned long long*)&s_mask[w + 1], (unsigned long long)gg >> (64 - off));
-        }
+        else
+            sg[r * 64 + lane] = gg;
     }
     if (!lds) return;
-    __syncthreads();
-    for (uint32_t i = tid; i < NMW; i += 256)
-        if (mw0 + i < nmw) st_nt(a.mask_out + mw0 + i, s_mask[i]);
+    const uint32_t nwm = R * eps; // the wave's mask words (<= 8*21)
+    for (uint32_t mw = lane; mw < nwm; mw += 64) {
+        const uint32_t wa = (mw * 64) / eps;    // the word holding the mask word's first element
+        const uint32_t sk = mw * 64 - wa * eps; // its elements before that one
+        uint64_t m = sg[wa] >> sk;
+        uint32_t w = wa + 1, sh = eps - sk;
+        // two words per step (independent loads); both start inside the
+        // mask word, so both exist
+        for (; sh + eps < 64; w += 2, sh += 2 * eps) {
+            const uint32_t g0 = sg[w], g1 = sg[w + 1];
+            m |= ((uint64_t)g0 << sh) | ((uint64_t)g1 << (sh + eps));
+        }
+        if (sh < 64) m |= (uint64_t)sg[w] << sh;
+        const uint64_t mi = mw0 + (uint64_t)wave * nwm + mw;
+        if (mi < nmw) st_nt(a.mask_out + mi, m); // bits past n are zero (cleared above)
+    }
 }
 
 // ---- sum: lane-per-word decode, per-block partials, then one final block.
@@ -462,18 +472,18 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
         a.pow2 = (a.eps & (a.eps - 1)) == 0;
         a.in_words = (c.n + a.eps - 1) / a.eps;
         // enough blocks to keep every CU busy to the end (~8192), at most
-        // FW_RMAX words per lane, and LDS mask words 4*R*eps <= FW_MASKW
+        // FW_RMAX words per lane
         uint64_t r = a.in_words / (256ull * 8192);
         if (r < 2) r = 2;
-        if (!a.pow2 && r > FW_MASKW / (4 * a.eps)) r = FW_MASKW / (4 * a.eps);
         a.R = (uint32_t)(r > FW_RMAX ? FW_RMAX : r);
         if (c.bits <= 32) a.tp = make_top_pred(pr, c.bits, a.eps);
         a.mask_out = mask_out;
         const uint64_t TW = 256ull * a.R;
         const uint64_t TWB = (a.in_words + TW - 1) / TW;
         if (TWB >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
-        if (c.bits > 32) hipLaunchKernelGGL(k_filter_words<true>, dim3((uint32_t)TWB), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_filter_words<false>, dim3((uint32_t)TWB), dim3(256), 0, s, a);
+        if (c.bits > 32) hipLaunchKernelGGL((k_filter_words<true, false>), dim3((uint32_t)TWB), dim3(256), 0, s, a);
+        else if (a.pow2) hipLaunchKernelGGL((k_filter_words<false, false>), dim3((uint32_t)TWB), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_filter_words<false, true>), dim3((uint32_t)TWB), dim3(256), 0, s, a);
         return check_launch();
     }
     if (c.kind == K_SLAB) c.kind = K_BITSTREAM; // 1-bit slabs are dense streams

@@ -227,6 +227,31 @@ def test_config_c1_exact(am, oracle_mod):
     ws.check()
 
 
+def test_filter_full_size_matches_compress_mask(am, oracle_mod):
+    """At 2^28 elements the word filter runs its widest tiles (8 words per
+    lane, mask words gathered per wave), which the small parity cases do not
+    reach. Size-independent check: its mask equals the mask the compaction
+    kernel writes for the same predicate (an independent kernel, pinned to the
+    oracle above), at the selectivity the predicate implies."""
+    import torch
+    n = 1 << 28
+    ws = am.Workspace(n)
+    for w in (3, 5, 7, 11, 17):
+        col = am.gen_uniform(7 + w, n, w, 1)
+        for op, c in ((0, 1 << (w - 1)), (3, (1 << w) // 3)):
+            say("filter-full", w, op, c)
+            fm = am.filter(col, op, c)
+            out, stats, mo = am.compressstore(col, 32, 4, pred=(op, c), want_mask=True, ws=ws)
+            torch.cuda.synchronize()
+            nmw = (n + 63) // 64
+            assert torch.equal(fm[:nmw], mo[:nmw]), (w, op, c)
+            k = int(am.u64(stats)[0])
+            frac = c / (1 << w) if op == 0 else ((1 << w) - c) / (1 << w)  # uniform w-bit values
+            assert abs(k - n * frac) < n // 50, (w, op, c, k)
+            del out, stats, mo, fm
+    ws.check()
+
+
 def test_compress_capacity_truncation(am, oracle_mod):
     rng = np.random.default_rng(5)
     ws = am.Workspace(1 << 16)
