@@ -91,7 +91,22 @@ __global__ __launch_bounds__(256) void k_filter_stream(Col in, Pred pr, uint64_t
     uint64_t ib1 = (tend * w + 63) >> 6;
     if (ib1 > in.words) ib1 = in.words;
     const uint32_t nin = (uint32_t)(ib1 - ib0);
-    for (uint32_t i = tid; i < FS_IN; i += 256) s_w[i] = i < nin ? in.p[ib0 + i] : 0;
+    {
+        // every load of the thread first (each LDS store would otherwise wait
+        // for its own load: one memory round trip per step)
+        constexpr uint32_t PER = (FS_IN + 255) / 256;
+        uint64_t v[PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t i = tid + 256 * k;
+            v[k] = i < nin ? in.p[ib0 + i] : 0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t i = tid + 256 * k;
+            if (i < FS_IN) s_w[i] = v[k];
+        }
+    }
     __syncthreads();
     const uint32_t* s32 = (const uint32_t*)s_w;
     const uint32_t rb = (uint32_t)(t0 * w - ib0 * 64); // input bit of element t0 in the staged run
