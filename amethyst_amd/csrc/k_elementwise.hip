@@ -213,9 +213,25 @@ __global__ __launch_bounds__(256) void k_binary_stream(BinStream g)
     uint64_t ib1 = (tend * wi + 63) >> 6;
     if (ib1 > g.words_i) ib1 = g.words_i;
     const uint32_t nin = (uint32_t)(ib1 - ib0); // <= BS_IN (wo >= wi)
-    for (uint32_t i = tid; i <= nin; i += 256) {
-        s_a[i] = i < nin ? g.a.p[ib0 + i] : 0;
-        s_b[i] = i < nin ? g.b.p[ib0 + i] : 0;
+    {
+        // every load of the thread first (each LDS store would otherwise wait
+        // for its own load: one memory round trip per step)
+        constexpr uint32_t PER = (BS_IN + 1 + 255) / 256;
+        uint64_t va[PER], vb[PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t i = tid + 256 * k;
+            va[k] = i < nin ? g.a.p[ib0 + i] : 0;
+            vb[k] = i < nin ? g.b.p[ib0 + i] : 0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            const uint32_t i = tid + 256 * k;
+            if (i <= nin) {
+                s_a[i] = va[k];
+                s_b[i] = vb[k];
+            }
+        }
     }
     __syncthreads();
     const uint32_t* sa = (const uint32_t*)s_a;
