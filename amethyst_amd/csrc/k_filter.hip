@@ -150,7 +150,10 @@ constexpr uint32_t FW_SELW = 4 * 64 * FW_RMAX; // LDS selection words (u32) per 
 template <bool WIDE, bool GATHER>
 __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
 {
-    __shared__ uint32_t s_sel[FW_SELW]; // non-power-of-two eps: selection bits per word
+    // non-power-of-two eps: selection bits per word (u32); otherwise the
+    // wave's mask words (u64, at most 32*R of them), stored together at the end
+    __shared__ uint64_t s_lds[FW_SELW / 2];
+    uint32_t* const s_sel = (uint32_t*)s_lds;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t t = blockIdx.x;
     const uint32_t fw = a.fw, eps = a.eps, R = a.R, bw = a.in.bits;
@@ -160,6 +163,7 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
     const uint64_t nmw = (n + 63) >> 6;
     const uint64_t mw0 = (w0 * eps) >> 6;
     uint32_t* const sg = s_sel + wave * 64 * R; // this wave's words
+    uint64_t* const sm = (uint64_t*)sg;         // (power-of-two eps) its mask words
     uint64_t x[FW_RMAX];
 #pragma unroll
     for (uint32_t r = 0; r < FW_RMAX; r++) {
@@ -182,20 +186,27 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
             const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
             gg &= (uint32_t)lowmask(nv);
         }
-        const uint64_t cmw = mw0 + (uint64_t)ci * eps; // first mask word of the chunk
         if (eps == 1) { // the chunk's mask word is the ballot
             const uint64_t m = ballot(gg & 1);
-            if (lane == 0 && cmw < nmw) st_nt(a.mask_out + cmw, m);
+            if (lane == 0) sm[r] = m;
         }
         else if (!lds) {
             const uint64_t m = segment_or((uint64_t)gg << __umul24(lane & (S - 1), eps), S);
-            const uint64_t mi = cmw + lane / S;
-            if ((lane & (S - 1)) == S - 1 && mi < nmw) st_nt(a.mask_out + mi, m);
+            if ((lane & (S - 1)) == S - 1) sm[r * eps + lane / S] = m;
         }
         else
             sg[r * 64 + lane] = gg;
     }
-    if (!lds) return;
+    if (!lds) {
+        // the wave's R*eps mask words are consecutive: one store of them all
+        // (per-chunk stores were 8-16 byte pieces of a line at eps = 1, 2)
+        const uint32_t nwm = R * eps;
+        for (uint32_t mw = lane; mw < nwm; mw += 64) {
+            const uint64_t mi = mw0 + (uint64_t)wave * nwm + mw;
+            if (mi < nmw) st_nt(a.mask_out + mi, sm[mw]);
+        }
+        return;
+    }
     const uint32_t nwm = R * eps; // the wave's mask words (<= 8*21)
     for (uint32_t mw = lane; mw < nwm; mw += 64) {
         const uint32_t wa = (mw * 64) / eps;    // the word holding the mask word's first element
