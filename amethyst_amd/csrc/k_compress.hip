@@ -373,14 +373,23 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
             if (lane == 63) s_ccnt[ci] = inc;
             if (PRED && a.mask_out) {
                 if (pow2) { // the lanes of one mask word are an aligned segment: DPP OR
+                    // (into this wave's consumed input words, stored together below)
                     const uint32_t S = 64 / eps;
                     const uint64_t m = eps == 1 ? ballot(gg & 1)
                                                 : segment_or((uint64_t)gg << __umul24(lane & (S - 1), eps), S);
-                    const uint64_t mi = mw0 + (uint64_t)ci * eps + lane / S;
-                    if ((lane & (S - 1)) == S - 1 && mi < nmw) st_nt(a.mask_out + mi, m);
+                    if ((lane & (S - 1)) == S - 1) s_x[wave * 64 * R + r * eps + lane / S] = m;
                 }
                 else // the selection bits, into this wave's own (consumed) input words
                     ((uint32_t*)(s_x + wave * 64 * R))[r * 64 + lane] = gg;
+            }
+        }
+        if (PRED && a.mask_out && pow2) {
+            // the wave's R*eps mask words are consecutive: one store of them
+            // all (per-chunk stores were 8-16 byte pieces of a line at eps = 1, 2)
+            const uint32_t nwm = R * eps;
+            for (uint32_t k = lane; k < nwm; k += 64) {
+                const uint64_t mi = mw0 + (uint64_t)wave * nwm + k;
+                if (mi < nmw) st_nt(a.mask_out + mi, s_x[wave * 64 * R + k]);
             }
         }
         if (PRED && a.mask_out && !pow2) {
