@@ -107,9 +107,52 @@ struct BinWords {
     Col a, b, out;
     uint32_t fi, eps_i, fo, eps_o;
     uint64_t words_i, words_o;
+    uint64_t fmask; // aligned bitwise ops at equal width: every field's bits
 };
 
-template <int OP, bool ALIGNED>
+// Aligned columns (input and output words hold the same elements): a thread
+// takes BA_PER words, loads them all before computing any (more bytes in
+// flight per wave than one word per thread). AND/OR/XOR at equal width work
+// on whole words under the fields' mask (dead bits stay zero).
+constexpr uint32_t BA_PER = 4;
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_binary_aligned(BinWords g)
+{
+    constexpr bool BITWISE = OP == AM_OP_AND || OP == AM_OP_OR || OP == AM_OP_XOR;
+    const uint64_t q0 = (uint64_t)blockIdx.x * (256 * BA_PER) + threadIdx.x;
+    const uint64_t n = g.out.n;
+    const uint32_t fi = g.fi, fo = g.fo, eo = g.eps_o;
+    const uint64_t im = g.a.mask, om = g.out.mask;
+    uint64_t x[BA_PER], y[BA_PER];
+#pragma unroll
+    for (uint32_t k = 0; k < BA_PER; k++) {
+        const uint64_t q = q0 + 256 * k;
+        x[k] = q < g.words_o ? __builtin_nontemporal_load(g.a.p + q) : 0;
+        y[k] = q < g.words_o ? __builtin_nontemporal_load(g.b.p + q) : 0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < BA_PER; k++) {
+        const uint64_t q = q0 + 256 * k;
+        if (q >= g.words_o) break;
+        const uint64_t e0 = q * eo;
+        const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
+        uint64_t word = 0;
+        if (BITWISE && g.fmask) {
+            const uint64_t m = cnt == eo ? g.fmask : g.fmask & lowmask(cnt * fi);
+            word = binop<OP>(x[k] & m, y[k] & m) & m;
+        }
+        else
+            for (uint32_t j = 0, si = 0, so = 0; j < eo; j++, si += fi, so += fo) {
+                const uint64_t r = binop<OP>((x[k] >> si) & im, (y[k] >> si) & im) & om;
+                word |= (j < cnt ? r : 0) << so;
+            }
+        st_nt(g.out.p + q, word);
+    }
+}
+
+// Unaligned columns: one output word per lane.
+template <int OP>
 __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
 {
     const uint64_t u0 = (uint64_t)blockIdx.x * 256;
@@ -121,14 +164,7 @@ __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
     const uint64_t e0 = q * eo;
     const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
     uint64_t word = 0;
-    if (ALIGNED) {
-        const uint64_t x = __builtin_nontemporal_load(g.a.p + q), y = __builtin_nontemporal_load(g.b.p + q);
-        for (uint32_t j = 0, si = 0, so = 0; j < eo; j++, si += fi, so += fo) {
-            const uint64_t r = binop<OP>((x >> si) & im, (y >> si) & im) & om;
-            word |= (j < cnt ? r : 0) << so;
-        }
-    }
-    else {
+    {
         // block base: element u0*eo lies in input word qb at field rb
         const uint64_t t0 = u0 * eo;
         const uint64_t qb = t0 / ei;
@@ -175,9 +211,15 @@ static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s
     g.a.eps_magic = g.eps_i > 1 ? (uint32_t)((1ull << 32) / g.eps_i + 1) : 0;
     g.words_i = (a.n + g.eps_i - 1) / g.eps_i;
     g.words_o = (o.n + g.eps_o - 1) / g.eps_o;
+    if (g.eps_o == g.eps_i) {
+        if (g.fi == g.fo && a.mask == o.mask) // equal field layout
+            for (uint32_t j = 0; j < g.eps_i; j++) g.fmask |= a.mask << (j * g.fi);
+        const dim3 grid((uint32_t)((g.words_o + 256 * BA_PER - 1) / (256 * BA_PER)));
+        hipLaunchKernelGGL((k_binary_aligned<OP>), grid, dim3(256), 0, s, g);
+        return true;
+    }
     const dim3 grid((uint32_t)((g.words_o + 255) / 256));
-    if (g.eps_o == g.eps_i) hipLaunchKernelGGL((k_binary_words<OP, true>), grid, dim3(256), 0, s, g);
-    else hipLaunchKernelGGL((k_binary_words<OP, false>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((k_binary_words<OP>), grid, dim3(256), 0, s, g);
     return true;
 }
 
