@@ -17,9 +17,9 @@ every plan on seeded columns, produced by this module's own sweep
 (`collect`); `evaluate` measures regret against the exhaustively best plan
 on held-out seeds.
 
-    python -m amethyst_amd.planner collect  --n 3000 --seed 1000 --out gpurun_out/plan_train.npz
+    python -m amethyst_amd.planner collect  --n 3000 --seed 1000 --out plan_train.npz
     python -m amethyst_amd.planner train    --data plan_train.npz --out amethyst_amd/planner_weights.npz
-    python -m amethyst_amd.planner evaluate --n 10000 --seed 900000 --out gpurun_out/plan_eval.json
+    python -m amethyst_amd.planner evaluate --n 10000 --seed 900000 --out plan_eval.json
 """
 import argparse
 import ctypes
@@ -431,7 +431,7 @@ def main(argv=None):
     c.add_argument("--seed", type=int, default=1000)
     c.add_argument("--nmax", type=int, default=1 << 26)
     c.add_argument("--reps", type=int, default=3)
-    c.add_argument("--out", default="gpurun_out/plan_train.npz")
+    c.add_argument("--out", default="plan_train.npz")
     t = sub.add_parser("train")
     t.add_argument("--data", required=True, help="one or more .npz files, comma-separated")
     t.add_argument("--epochs", type=int, default=400)
@@ -443,7 +443,7 @@ def main(argv=None):
     e.add_argument("--reps", type=int, default=2)
     e.add_argument("--margin", type=float, default=None)
     e.add_argument("--weights", default=WEIGHTS)
-    e.add_argument("--out", default="gpurun_out/plan_eval.json")
+    e.add_argument("--out", default="plan_eval.json")
     args = ap.parse_args(argv)
     {"collect": _collect, "train": _train, "evaluate": _evaluate}[args.cmd](args)
 

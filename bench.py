@@ -351,8 +351,8 @@ def batched_leg(am, torch, reps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the operators are single-GPU; sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the operators run on a single GPU)", file=sys.stderr)
         sys.exit(2)
     import torch
     import amethyst_amd as am
