@@ -151,41 +151,56 @@ __global__ __launch_bounds__(256) void k_binary_aligned(BinWords g)
     }
 }
 
-// Unaligned columns: one output word per lane.
+// Unaligned columns: a thread builds BA_PER output words (one per 256-word
+// sub-block of its block), loading the input words of all of them before
+// combining any.
 template <int OP>
 __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
 {
-    const uint64_t u0 = (uint64_t)blockIdx.x * 256;
-    const uint64_t q = u0 + threadIdx.x;
-    if (q >= g.words_o) return;
     const uint64_t n = g.out.n;
     const uint32_t fi = g.fi, fo = g.fo, ei = g.eps_i, eo = g.eps_o;
     const uint64_t im = g.a.mask, om = g.out.mask;
-    const uint64_t e0 = q * eo;
-    const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
-    uint64_t word = 0;
-    {
-        // block base: element u0*eo lies in input word qb at field rb
+    uint64_t x0[BA_PER], y0[BA_PER], x1[BA_PER], y1[BA_PER];
+    uint32_t s0[BA_PER];
+#pragma unroll
+    for (uint32_t k = 0; k < BA_PER; k++) {
+        // sub-block base: element u0*eo lies in input word qb at field rb
+        const uint64_t u0 = ((uint64_t)blockIdx.x * BA_PER + k) * 256;
+        const uint64_t q = u0 + threadIdx.x;
+        x0[k] = y0[k] = x1[k] = y1[k] = 0;
+        s0[k] = 0;
+        if (q >= g.words_o) continue;
         const uint64_t t0 = u0 * eo;
         const uint64_t qb = t0 / ei;
         const uint32_t rb = (uint32_t)(t0 - qb * ei);
         const uint32_t rel = threadIdx.x * eo + rb; // < 2^24
         const uint32_t dq = ei == 1 ? rel : __umulhi(rel, g.a.eps_magic);
-        const uint32_t s0 = rel - dq * ei;
+        s0[k] = rel - dq * ei;
         const uint64_t qi = qb + dq;
-        const uint64_t x0 = g.a.p[qi], y0 = g.b.p[qi];
-        const bool two = s0 + eo > ei && qi + 1 < g.words_i;
-        const uint64_t x1 = two ? g.a.p[qi + 1] : 0, y1 = two ? g.b.p[qi + 1] : 0;
+        x0[k] = g.a.p[qi];
+        y0[k] = g.b.p[qi];
+        if (s0[k] + eo > ei && qi + 1 < g.words_i) {
+            x1[k] = g.a.p[qi + 1];
+            y1[k] = g.b.p[qi + 1];
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < BA_PER; k++) {
+        const uint64_t q = ((uint64_t)blockIdx.x * BA_PER + k) * 256 + threadIdx.x;
+        if (q >= g.words_o) break;
+        const uint64_t e0 = q * eo;
+        const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
+        uint64_t word = 0;
         for (uint32_t j = 0, so = 0; j < eo; j++, so += fo) {
-            const uint32_t s = s0 + j;
+            const uint32_t s = s0[k] + j;
             const bool hi = s >= ei;
             const uint32_t sh = (hi ? s - ei : s) * fi;
-            const uint64_t va = ((hi ? x1 : x0) >> sh) & im, vb = ((hi ? y1 : y0) >> sh) & im;
+            const uint64_t va = ((hi ? x1[k] : x0[k]) >> sh) & im, vb = ((hi ? y1[k] : y0[k]) >> sh) & im;
             const uint64_t r = binop<OP>(va, vb) & om;
             word |= (j < cnt ? r : 0) << so;
         }
+        st_nt(g.out.p + q, word);
     }
-    st_nt(g.out.p + q, word);
 }
 
 template <int OP>
@@ -218,7 +233,7 @@ static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s
         hipLaunchKernelGGL((k_binary_aligned<OP>), grid, dim3(256), 0, s, g);
         return true;
     }
-    const dim3 grid((uint32_t)((g.words_o + 255) / 256));
+    const dim3 grid((uint32_t)((g.words_o + 256 * BA_PER - 1) / (256 * BA_PER)));
     hipLaunchKernelGGL((k_binary_words<OP>), grid, dim3(256), 0, s, g);
     return true;
 }
