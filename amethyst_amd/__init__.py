@@ -30,7 +30,9 @@ LT, LE, GT, GE, EQ, NE = range(6)
 GB_EXACT, GB_REF = 0, 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported configuration", 3: "HIP runtime error",
-          4: "workspace missing or too small", 5: "output capacity too small"}
+          4: "workspace missing or too small", 5: "output capacity too small",
+          6: "look-back wait timed out (workspace poisoned until re-initialised)",
+          7: "input exceeds the no-allocation entry point's fixed scratch"}
 
 
 class AmethystError(RuntimeError):
@@ -98,6 +100,7 @@ def lib():
             "am_convert": ([colp, colp, vp], i32),
             "am_filter": ([colp, am_predicate, vp, vp], i32),
             "am_compressstore": ([colp, vp, predp, colp, vp, vp, vp, u64, vp], i32),
+            "am_compressstore_noalloc": ([colp, vp, predp, colp, vp, vp, vp], i32),
             "am_binary": ([u32, colp, colp, colp, vp], i32),
             "am_sum": ([colp, vp, vp, u64, vp], i32),
             "am_groupby": ([colp, colp, predp, u32, vp, vp, vp, vp, vp, u64, vp], i32),

@@ -468,7 +468,7 @@ struct Lookback {
     uint32_t epoch, bits;
     WsHeader* hdr;
 
-    __host__ __device__ static uint64_t bytes_for(uint64_t tiles_cap)
+    __host__ __device__ static constexpr uint64_t bytes_for(uint64_t tiles_cap)
     {
         return tiles_cap * TS * 8 + (tiles_cap / 64 + 2) * GS * 8;
     }

@@ -29,6 +29,7 @@ extern "C" const char* am_status_string(int status)
         case AM_E_WORKSPACE: return "workspace missing or too small";
         case AM_E_CAPACITY: return "output capacity too small";
         case AM_E_ABORTED: return "look-back wait timed out; workspace poisoned until re-initialised";
+        case AM_E_SCRATCH_LIMIT: return "input exceeds the fixed scratch of the no-allocation entry point";
     }
     return "unknown status";
 }

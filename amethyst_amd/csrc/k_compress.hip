@@ -30,6 +30,8 @@
 //      no read-modify-write.
 // Descriptor words carry the launch epoch (bumped by the last tile), so the
 // workspace needs no clearing between launches or graph replays.
+#include <atomic>
+#include <mutex>
 #include <type_traits>
 
 #include "am_device.hpp"
@@ -757,8 +759,11 @@ uint64_t am_compress_desc_bytes(uint64_t n)
     return Lookback<LB_WIN | LB_OVF>::bytes_for(tiles);
 }
 
-extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const am_predicate* pred, am_column* out,
-                                uint64_t* mask_out, uint64_t* stats_dev, void* ws, uint64_t ws_bytes, void* stream)
+// One compaction launch with its look-back descriptors at `desc` (`desc_bytes`
+// bytes) and the launch epoch / abort word at `hdr`.
+static int compress_launch(const am_column* in, const uint64_t* mask, const am_predicate* pred, am_column* out,
+                           uint64_t* mask_out, uint64_t* stats_dev, WsHeader* hdr, uint8_t* desc, uint64_t desc_bytes,
+                           hipStream_t s)
 {
     Col ci, co;
     int st = make_col(in, &ci);
@@ -766,7 +771,6 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     if ((st = make_col(out, &co))) return st;
     if (!mask && !pred) return AM_E_INVALID;
     if (pred && pred->op > AM_PRED_NE) return AM_E_INVALID;
-    if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
     if (out->n && !out->data) return AM_E_INVALID;
     CompressArgs a{};
     a.in = ci;
@@ -776,7 +780,7 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     a.cap = out->n;
     a.mask_out = mask_out;
     a.stats = stats_dev;
-    a.hdr = (WsHeader*)ws;
+    a.hdr = hdr;
     if (ci.kind == K_SLAB && ci.bits == 1) ci.kind = K_BITSTREAM; // 64 one-bit fields per word = dense stream
     a.in = ci;
     const bool words = ci.kind != K_BITSTREAM; // word kernel: at most 32 fields per word
@@ -806,19 +810,17 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     }
     const bool ovf = co.bits < ci.bits;
     const uint32_t F = cs_flags(co.kind, ovf);
-    uint8_t* base = (uint8_t*)ws + WS_FIXED;
-    const uint64_t bytes = ws_bytes - WS_FIXED;
     uint64_t cap;
     if (F == 0) {
-        const auto L = Lookback<0>::at(base, bytes);
+        const auto L = Lookback<0>::at(desc, desc_bytes);
         a.lb_tiles = L.tiles, a.lb_groups = L.groups, cap = L.cap;
     }
     else if (F == LB_OVF) {
-        const auto L = Lookback<LB_OVF>::at(base, bytes);
+        const auto L = Lookback<LB_OVF>::at(desc, desc_bytes);
         a.lb_tiles = L.tiles, a.lb_groups = L.groups, cap = L.cap;
     }
     else {
-        const auto L = Lookback<LB_WIN | LB_OVF>::at(base, bytes);
+        const auto L = Lookback<LB_WIN | LB_OVF>::at(desc, desc_bytes);
         a.lb_tiles = L.tiles, a.lb_groups = L.groups, cap = L.cap;
     }
     if (T > cap || T >= (1ull << 31)) return AM_E_WORKSPACE;
@@ -826,11 +828,99 @@ extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const
     a.T = (uint32_t)T;
     const uint32_t vb = value_bytes(ci.bits < co.bits ? ci.bits : co.bits);
     const bool wide = ci.bits > 32;
-    hipStream_t s = (hipStream_t)stream;
     switch (co.kind) {
         case K_STRIDED: launch_out<K_STRIDED>(a, words, pred != nullptr && !mask, ovf, vb, wide, s); break;
         case K_SLAB: launch_out<K_SLAB>(a, words, pred != nullptr && !mask, ovf, vb, wide, s); break;
         default: launch_out<K_BITSTREAM>(a, words, pred != nullptr && !mask, ovf, vb, wide, s); break;
     }
     return check_launch();
+}
+
+extern "C" int am_compressstore(const am_column* in, const uint64_t* mask, const am_predicate* pred, am_column* out,
+                                uint64_t* mask_out, uint64_t* stats_dev, void* ws, uint64_t ws_bytes, void* stream)
+{
+    if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
+    return compress_launch(in, mask, pred, out, mask_out, stats_dev, (WsHeader*)ws, (uint8_t*)ws + WS_FIXED,
+                           ws_bytes - WS_FIXED, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------ no-allocation entry point
+//
+// am_compressstore_noalloc keeps its launch epoch and look-back descriptors
+// in `scratch`, a file-scope device array (one copy per GPU, alive until the
+// process ends). Host code never passes the array's name to a kernel: its
+// device address comes from hipGetSymbolAddress, once per device, and is
+// kept in na_ptr[device].
+namespace am {
+
+constexpr uint64_t NA_TILES = AM_NOALLOC_MAX_ELEMENTS / 1024 + 2; // tiles hold at least 1024 elements
+constexpr uint64_t NA_DESC_BYTES = Lookback<LB_WIN | LB_OVF>::bytes_for(NA_TILES);
+constexpr uint64_t NA_BYTES = sizeof(WsHeader) + NA_DESC_BYTES;
+static_assert(NA_BYTES <= AM_NOALLOC_SCRATCH_BYTES, "scratch size stated in amethyst.h");
+constexpr uint64_t NA_WORDS = AM_NOALLOC_SCRATCH_BYTES / 8;
+constexpr int NA_MAX_DEVICES = 64;
+
+static __device__ __align__(256) uint64_t scratch[NA_WORDS];
+
+static std::atomic<uint64_t*> na_ptr[NA_MAX_DEVICES];
+static std::mutex na_mu;
+
+__global__ __launch_bounds__(256) void k_na_clear(uint64_t* p, uint64_t words)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// One block: if an earlier call gave up waiting (abort word set), zero the
+// whole scratch -- header included -- so this call starts from a clean
+// state; otherwise do nothing.
+__global__ __launch_bounds__(256) void k_na_heal(uint64_t* p, uint64_t words)
+{
+    __shared__ uint32_t s_ab;
+    if (threadIdx.x == 0) s_ab = ld32(&((WsHeader*)p)->abort);
+    __syncthreads();
+    if (!s_ab) return;
+    for (uint64_t i = threadIdx.x; i < words; i += 256) p[i] = 0;
+}
+
+// Device address of this device's scratch; the first call on a device looks
+// it up and queues its zeroing on `s`.
+static int na_scratch(hipStream_t s, uint64_t** out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return AM_E_HIP;
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return AM_E_UNSUPPORTED_CONFIG;
+    uint64_t* p = na_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        std::lock_guard<std::mutex> g(na_mu);
+        p = na_ptr[dev].load(std::memory_order_relaxed);
+        if (!p) {
+            void* q = nullptr;
+            if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return AM_E_HIP;
+            p = (uint64_t*)q;
+            hipLaunchKernelGGL(k_na_clear, dim3(1024), dim3(256), 0, s, p, NA_WORDS);
+            if (hipGetLastError() != hipSuccess) return AM_E_HIP;
+            na_ptr[dev].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return AM_OK;
+}
+
+} // namespace am
+
+extern "C" int am_compressstore_noalloc(const am_column* in, const uint64_t* mask, const am_predicate* pred,
+                                        am_column* out, uint64_t* mask_out, uint64_t* stats_dev, void* stream)
+{
+    if (!in || !out) return AM_E_INVALID;
+    if (in->n > AM_NOALLOC_MAX_ELEMENTS) return AM_E_SCRATCH_LIMIT;
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t* p = nullptr;
+    int st = na_scratch(s, &p);
+    if (st) return st;
+    hipLaunchKernelGGL(k_na_heal, dim3(1), dim3(256), 0, s, p, NA_WORDS);
+    if (hipGetLastError() != hipSuccess) return AM_E_HIP;
+    st = compress_launch(in, mask, pred, out, mask_out, stats_dev, (WsHeader*)p, (uint8_t*)p + sizeof(WsHeader),
+                         AM_NOALLOC_SCRATCH_BYTES - sizeof(WsHeader), s);
+    return st == AM_E_WORKSPACE ? AM_E_SCRATCH_LIMIT : st;
 }

@@ -2,10 +2,11 @@
  * amethyst.h -- C ABI of the MI355X (gfx950) packed-column operator engine.
  *
  * Every entry point takes plain device pointers, sizes and a HIP stream
- * passed as `void*` (a hipStream_t; NULL = the default stream). Nothing here
- * allocates device memory except the helpers named *_create; every operator
- * is stream-ordered and asynchronous, reentrant, and keeps no global state.
- * Errors are returned as AM_* status codes; nothing aborts the process.
+ * passed as `void*` (a hipStream_t; NULL = the default stream). No operator
+ * allocates device memory; every operator is stream-ordered and asynchronous
+ * and reentrant. The only global state is the fixed per-GPU scratch array of
+ * am_compressstore_noalloc. Errors are returned as AM_* status codes; nothing
+ * aborts the process.
  *
  * The reference (yogi-tud/Amethyst) is a CUDA benchmark whose operators are
  * templated kernels `kernel_<op>_<format><block_size, bits, [outbits], op>`
@@ -55,8 +56,10 @@ typedef enum {
     AM_E_HIP = 3,                /* a HIP runtime call failed */
     AM_E_WORKSPACE = 4,          /* workspace missing or too small */
     AM_E_CAPACITY = 5,           /* output buffer too small for the request */
-    AM_E_ABORTED = 6             /* a look-back wait exceeded its time limit; the
+    AM_E_ABORTED = 6,            /* a look-back wait exceeded its time limit; the
                                     workspace is poisoned until re-initialised */
+    AM_E_SCRATCH_LIMIT = 7       /* input larger than am_compressstore_noalloc's
+                                    fixed device scratch (AM_NOALLOC_MAX_ELEMENTS) */
 } am_status;
 
 typedef enum {
@@ -223,6 +226,23 @@ int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_out, void* 
 int am_compressstore(const am_column* in, const uint64_t* mask, const am_predicate* pred,
                      am_column* out, uint64_t* mask_out, uint64_t* stats_dev,
                      void* ws, uint64_t ws_bytes, void* stream);
+
+/* am_compressstore without a workspace and without any allocation, for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc. No counterpart in the
+ * reference. Same arguments, semantics and results as am_compressstore; its
+ * look-back descriptors live in a fixed device array of
+ * AM_NOALLOC_SCRATCH_BYTES bytes that the library holds on each GPU it runs
+ * on until the process ends (looked up and zeroed on the first call on a
+ * device, which must be the caller's current device). Inputs longer than
+ * AM_NOALLOC_MAX_ELEMENTS return AM_E_SCRATCH_LIMIT. Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (two streams, two
+ * host threads) race and give wrong results -- order them on one stream.
+ * Calls no hipMalloc/hipFree. */
+#define AM_NOALLOC_MAX_ELEMENTS (1ull << 28)
+#define AM_NOALLOC_SCRATCH_BYTES ((10ull << 20) + (256ull << 10))
+int am_compressstore_noalloc(const am_column* in, const uint64_t* mask, const am_predicate* pred,
+                             am_column* out, uint64_t* mask_out, uint64_t* stats_dev, void* stream);
 
 /* Element-wise out[i] = op(a[i], b[i]) & mask(out.bits); a and b share n and
  * bits; any input format x any output format.

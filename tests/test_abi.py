@@ -78,9 +78,9 @@ def test_status_strings_and_errors(L):
     import amethyst_amd as am
     assert L.am_status_string(0) == b"ok"
     assert L.am_status_string(2) == b"unsupported configuration"
-    for code in range(7):  # every am_status has its own message
+    for code in range(8):  # every am_status has its own message
         assert L.am_status_string(code) != b"unknown status", code
-    assert L.am_status_string(7) == b"unknown status"
+    assert L.am_status_string(8) == b"unknown status"
     assert L.am_workspace_status(None) == 1  # null workspace: invalid, checked before any HIP call
     # invalid arguments are rejected on the host before any launch
     bad = am.am_column(None, 10, 11, am.SLAB)
