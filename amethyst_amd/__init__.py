@@ -96,6 +96,8 @@ def lib():
             "am_gen_mask_bernoulli": ([u64, ctypes.c_double, vp, u64, vp], i32),
             "am_gen_mask_runs": ([u64, ctypes.c_double, ctypes.c_double, vp, u64, vp], i32),
             "am_gen_sorted_noise": ([u64, u32, colp, vp], i32),
+            "am_gen_mask_uniform_ref": ([u64, ctypes.c_double, vp, u64, u64, vp], i32),
+            "am_gen_mask_clusters_ref": ([ctypes.c_float, u64, u64, vp, vp], i32),
             "am_stream_copy": ([vp, vp, u64, vp], i32),
             "am_convert": ([colp, colp, vp], i32),
             "am_filter": ([colp, am_predicate, vp, vp], i32),
@@ -254,6 +256,32 @@ def gen_mask_bernoulli(seed, n, selectivity, stream=None):
     m = device_words((n + 63) // 64)
     _check(lib().am_gen_mask_bernoulli(seed, selectivity, ctypes.c_void_p(m.data_ptr()), n, _stream(stream)),
            "am_gen_mask_bernoulli")
+    return m
+
+
+def gen_sorted_noise(seed, n, bits, fmt, noise_bits, stream=None):
+    """Ascending values with noise (the uneven batch's sorted columns)."""
+    col = Column(n, bits, fmt)
+    c = col.c()
+    _check(lib().am_gen_sorted_noise(seed, noise_bits, ctypes.byref(c), _stream(stream)), "am_gen_sorted_noise")
+    return col
+
+
+def gen_mask_uniform_ref(nbytes, selectivity, seed=42, stream=None):
+    """The reference's generate_mask_uniform bytes (MSB-first within a byte)
+    as a device buffer of ceil(nbytes/8) words."""
+    m = device_words((nbytes + 7) // 8)
+    _check(lib().am_gen_mask_uniform_ref(seed, selectivity, ctypes.c_void_p(m.data_ptr()), 0, nbytes,
+                                         _stream(stream)), "am_gen_mask_uniform_ref")
+    return m
+
+
+def gen_mask_clusters_ref(selectivity, clusters, total, stream=None):
+    """The reference's create_bitmask bytes (total/8 of them) as a device
+    buffer of ceil(total/64) words (bytes past total/8 zero)."""
+    m = device_words((total + 63) // 64)
+    _check(lib().am_gen_mask_clusters_ref(selectivity, clusters, total, ctypes.c_void_p(m.data_ptr()),
+                                          _stream(stream)), "am_gen_mask_clusters_ref")
     return m
 
 

@@ -258,6 +258,39 @@ __global__ __launch_bounds__(256) void k_gen_mask_runs(uint64_t s0, uint64_t p10
     }
 }
 
+// generate_mask_uniform (src/main.cu:33-46), one lane per byte: byte i of
+// the run gets bit j (7 down to 0, MSB first) set iff draw 8i + (7-j) <
+// p_adj, p_adj = (uint64)(selectivity * UINT32_MAX).
+__global__ __launch_bounds__(256) void k_gen_mask_uniform_ref(uint64_t s0, uint64_t p_adj, uint8_t* buf,
+                                                              uint64_t length)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= length) return;
+    uint64_t st = state_at(s0, 8 * i);
+    uint32_t acc = 0;
+    for (int j = 7; j >= 0; j--) {
+        if ((uint64_t)pcg_out(st) < p_adj) acc |= 1u << j;
+        st = PCG_MULT * st + PCG_INC;
+    }
+    buf[i] = (uint8_t)acc;
+}
+
+// create_bitmask (src/main.cu:59-108), one lane per byte: bit b is set iff
+// it lies in the first `cluster` bits of one of the first `count` slices of
+// `slice` bits; bit b is bit 7 - b%8 of byte b/8.
+__global__ __launch_bounds__(256) void k_gen_mask_clusters_ref(uint64_t cluster, uint64_t slice, uint64_t count,
+                                                               uint8_t* out, uint64_t bytes)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bytes) return;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < 8; k++) {
+        const uint64_t b = 8 * i + k;
+        if (b / slice < count && b % slice < cluster) acc |= 1u << (7 - k);
+    }
+    out[i] = (uint8_t)acc;
+}
+
 } // namespace am
 
 using namespace am;
@@ -356,6 +389,32 @@ extern "C" int am_gen_mask_runs(uint64_t seed, double selectivity, double mean_r
     uint64_t chunks = (words + chunk_words - 1) / chunk_words;
     hipLaunchKernelGGL(k_gen_mask_runs, dim3((uint32_t)((chunks + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        splitmix_seed(seed), thr(p10), thr(p01), thr(selectivity), mask, n, chunk_words);
+    return check_launch();
+}
+
+extern "C" int am_gen_mask_uniform_ref(uint64_t seed, double selectivity, uint8_t* buf, uint64_t offset,
+                                       uint64_t length, void* stream)
+{
+    if (!buf && length) return AM_E_INVALID;
+    if (length == 0) return AM_OK;
+    const uint64_t p_adj = (uint64_t)(selectivity * (double)UINT32_MAX);
+    hipLaunchKernelGGL(k_gen_mask_uniform_ref, dim3((uint32_t)((length + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, splitmix_seed(seed), p_adj, buf + offset, length);
+    return check_launch();
+}
+
+extern "C" int am_gen_mask_clusters_ref(float selectivity, uint64_t cluster_count, uint64_t total, uint8_t* out,
+                                        void* stream)
+{
+    const uint64_t bytes = total / 8;
+    if ((!out && bytes) || cluster_count == 0) return AM_E_INVALID;
+    if (bytes == 0) return AM_OK;
+    // the reference forms selectivity * total in float
+    const uint64_t set_one = (uint64_t)(selectivity * (float)total);
+    uint64_t cluster = set_one / cluster_count, slice = total / cluster_count, count = cluster_count;
+    if (slice == 0) slice = 1, count = 0; // fewer bits than clusters: the reference sets none
+    hipLaunchKernelGGL(k_gen_mask_clusters_ref, dim3((uint32_t)((bytes + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, cluster, slice, count, out, bytes);
     return check_launch();
 }
 

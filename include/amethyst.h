@@ -173,9 +173,10 @@ int am_gen_uniform(uint64_t seed, am_column* out, void* stream);
  * used for Zipf-distributed group-by keys. No reference counterpart. */
 int am_gen_from_cdf(uint64_t seed, const uint32_t* thresholds_dev, uint32_t nbins, am_column* out, void* stream);
 
-/* Ascending values with noise: v_i = min(mask, floor(i*mask/n) + (draw(i) &
- * (2^noise_bits - 1))) -- the "sorted with noise" columns of the uneven
- * batch workload. No reference counterpart. */
+/* Ascending values with noise: v_i = min(mask, floor((i/n) * mask) +
+ * (draw(i) & (2^noise_bits - 1))), i/n and the product each rounded in IEEE
+ * double -- the "sorted with noise" columns of the uneven batch workload.
+ * No reference counterpart. */
 int am_gen_sorted_noise(uint64_t seed, uint32_t noise_bits, am_column* out, void* stream);
 
 /* Bernoulli(selectivity) mask: bit i = draw(i) < round(s * 2^32). LSB-first
@@ -188,6 +189,20 @@ int am_gen_mask_bernoulli(uint64_t seed, double selectivity, uint64_t* mask, uin
  * restarted from its stationary distribution every 4096 elements.
  * Counterpart of create_bitmask's clustered masks (src/main.cu:59-108). */
 int am_gen_mask_runs(uint64_t seed, double selectivity, double mean_run, uint64_t* mask, uint64_t n, void* stream);
+
+/* The reference driver's own mask generators, byte for byte, on the GPU.
+ * generate_mask_uniform (src/main.cu:33-46): bytes buf[offset ..
+ * offset+length) from a generator seeded `seed` (42 in the reference), bit j
+ * of each byte (7 down to 0: MSB first) set iff rand() <
+ * (uint64)(selectivity * UINT32_MAX). create_bitmask (src/main.cu:59-108):
+ * total/8 bytes with `cluster_count` runs of
+ * (size_t)(selectivity*total)/cluster_count ones (product formed in float),
+ * run i starting at bit i*(total/cluster_count), MSB first within a byte.
+ * The reference's kernels read these bytes LSB-first, as the operators here
+ * do, so passing the buffer as a mask reproduces its selections. */
+int am_gen_mask_uniform_ref(uint64_t seed, double selectivity, uint8_t* buf, uint64_t offset, uint64_t length,
+                            void* stream);
+int am_gen_mask_clusters_ref(float selectivity, uint64_t cluster_count, uint64_t total, uint8_t* out, void* stream);
 
 /* Copy nwords 64-bit words (16-byte aligned buffers) with 16-byte
  * non-temporal loads and stores: the device's sustained read+write rate,

@@ -46,6 +46,10 @@ def lib():
         L.or_gen_main_words.argtypes = [ctypes.c_uint64, _u64p, _u64p, ctypes.c_size_t]
         L.or_mask_uniform_ref.argtypes = [_u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double]
         L.or_mask_clusters_ref.argtypes = [_u8p, ctypes.c_float, ctypes.c_size_t, ctypes.c_size_t]
+        L.or_gen_mask_bernoulli.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64, _u64p]
+        L.or_gen_mask_runs.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _u64p]
+        L.or_gen_from_cdf.argtypes = [ctypes.c_uint64, _u32p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, _u64p]
+        L.or_gen_sorted_noise.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _u64p]
         L.or_mask.argtypes = [ctypes.c_int]
         L.or_mask.restype = ctypes.c_uint64
         L.or_packed_words.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
@@ -134,6 +138,31 @@ def mask_clusters_ref(selectivity, clusters, total):
     out = np.zeros(total // 8, dtype=np.uint8)
     lib().or_mask_clusters_ref(_p(out, _u8p), selectivity, clusters, total)
     return out
+
+
+def gen_mask_bernoulli(seed, n, selectivity):
+    out = np.zeros(max(1, (n + 63) // 64), dtype=np.uint64)
+    lib().or_gen_mask_bernoulli(seed, selectivity, n, _p(out, _u64p))
+    return out[:(n + 63) // 64]
+
+
+def gen_mask_runs(seed, n, selectivity, mean_run=256.0):
+    out = np.zeros(max(1, (n + 63) // 64), dtype=np.uint64)
+    lib().or_gen_mask_runs(seed, selectivity, mean_run, n, _p(out, _u64p))
+    return out[:(n + 63) // 64]
+
+
+def gen_from_cdf(seed, thresholds, bits, n):
+    thr = np.ascontiguousarray(thresholds, dtype=np.uint32)
+    out = np.empty(max(1, n), dtype=np.uint64)
+    lib().or_gen_from_cdf(seed, _p(thr, _u32p), len(thr), bits, n, _p(out, _u64p))
+    return out[:n]
+
+
+def gen_sorted_noise(seed, bits, noise_bits, n):
+    out = np.empty(max(1, n), dtype=np.uint64)
+    lib().or_gen_sorted_noise(seed, bits, noise_bits, n, _p(out, _u64p))
+    return out[:n]
 
 
 # --------------------------------------------------------------------- formats

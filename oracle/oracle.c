@@ -125,6 +125,85 @@ void or_mask_clusters_ref(uint8_t* out, float selectivity, size_t cluster_count,
     }
 }
 
+/* ------------------------------------------------ this build's generators
+ *
+ * The synthetic inputs of the benchmark configurations that the reference
+ * does not generate (its generators are the two above). Each restates, one
+ * draw at a time from the serial fast_prng stream, the algorithm that the
+ * matching am_gen_* entry point documents (include/amethyst.h); the device
+ * code evaluates the same draws in parallel with jump-ahead. */
+
+uint64_t or_mask(int bits);
+
+static uint64_t thr32(double p) /* round(p * 2^32), clamped to [0, 2^32] */
+{
+    double t = p * 4294967296.0;
+    return t <= 0 ? 0 : (t >= 4294967296.0 ? (1ull << 32) : (uint64_t)(t + 0.5));
+}
+
+/* am_gen_mask_bernoulli: bit i (LSB-first) = draw(i) < round(s * 2^32). */
+void or_gen_mask_bernoulli(uint64_t seed, double s, uint64_t n, uint64_t* out)
+{
+    or_rng r;
+    or_srand(&r, seed);
+    uint64_t thr = thr32(s);
+    memset(out, 0, ((n + 63) / 64) * 8);
+    for (uint64_t i = 0; i < n; i++)
+        if ((uint64_t)or_rand(&r) < thr) out[i / 64] |= 1ull << (i % 64);
+}
+
+/* am_gen_mask_runs: the bits are cut into chunks of 4096; chunk c uses draws
+ * c*4097 .. : its first draw starts the chain in state 1 iff draw <
+ * round(s*2^32); then per bit the current state is written and one draw
+ * moves the chain (state 1 leaves with probability 1/mean_run, state 0
+ * with s/(mean_run*(1-s)), thresholds rounded to 32-bit fractions). */
+void or_gen_mask_runs(uint64_t seed, double s, double mean_run, uint64_t n, uint64_t* out)
+{
+    double p10 = 1.0 / mean_run, p01 = s / (mean_run * (1.0 - s));
+    if (p01 > 1.0) p01 = 1.0;
+    uint64_t t10 = (uint64_t)(p10 * 4294967296.0 + 0.5), t01 = (uint64_t)(p01 * 4294967296.0 + 0.5);
+    uint64_t tpi = (uint64_t)(s * 4294967296.0 + 0.5);
+    memset(out, 0, ((n + 63) / 64) * 8);
+    or_rng r;
+    or_srand(&r, seed);
+    int on = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        if (i % 4096 == 0) on = (uint64_t)or_rand(&r) < tpi;
+        if (on) out[i / 64] |= 1ull << (i % 64);
+        uint64_t d = or_rand(&r);
+        on = on ? !(d < t10) : (d < t01);
+    }
+}
+
+/* am_gen_from_cdf: v_i = the first b with draw(i) < thr[b] (thr ascending,
+ * the last 2^32-1 so every draw but 2^32-1 itself maps below nb), masked. */
+void or_gen_from_cdf(uint64_t seed, const uint32_t* thr, uint32_t nb, int bits, uint64_t n, uint64_t* vals)
+{
+    or_rng r;
+    or_srand(&r, seed);
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t d = or_rand(&r), b = 0;
+        while (b + 1 < nb && !(d < thr[b])) b++;
+        vals[i] = (uint64_t)b & or_mask(bits);
+    }
+}
+
+/* am_gen_sorted_noise: v_i = min(mask, floor((i / n) * mask) + (draw(i) &
+ * (2^noise_bits - 1))), the quotient and product in IEEE double. */
+void or_gen_sorted_noise(uint64_t seed, int bits, int noise_bits, uint64_t n, uint64_t* vals)
+{
+    uint64_t m = or_mask(bits);
+    uint64_t nm = noise_bits <= 0 ? 0 : (noise_bits >= 32 ? 0xFFFFFFFFull : ((1ull << noise_bits) - 1));
+    or_rng r;
+    or_srand(&r, seed);
+    for (uint64_t i = 0; i < n; i++) {
+        volatile double frac = (double)i / (double)n; /* two separately rounded operations */
+        uint64_t base = (uint64_t)(frac * (double)m);
+        uint64_t v = base + ((uint64_t)or_rand(&r) & nm);
+        vals[i] = v > m ? m : v;
+    }
+}
+
 /* --------------------------------------------------------------- formats */
 
 /* gen_bitmask, src/common_code.cuh:38-41 (bits in 1..64). */

@@ -125,7 +125,11 @@ def _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, mask_words=None, pre
     st = am.u64(stats)
     assert int(st[0]) == k
     assert int(st[1]) == ovf
-    assert np.array_equal(am.u64(mo)[:(n + 63) // 64], mask_words)
+    # the mask written back has the input's selection; bits past n are zero
+    exp_mask = np.array(mask_words[:(n + 63) // 64], dtype=np.uint64)
+    if n % 64:
+        exp_mask[-1] &= np.uint64((1 << (n % 64)) - 1)
+    assert np.array_equal(am.u64(mo)[:(n + 63) // 64], exp_mask)
     # words the oracle defines: those holding elements < min(k, cap)
     kk = min(k, out.n)
     used = oracle_mod.packed_words(fo, ob, kk)
@@ -452,8 +456,9 @@ def test_matmul_packed(am, oracle_mod):
 
 
 def test_matmul_single_large(am, oracle_mod):
-    """One matmul with M, N, K >= 512 (the library int8 GEMM + row packer path),
-    with and without the int32 copy; ragged N and K."""
+    """One matmul with M, N, K >= 512 (the MFMA GEMM with the fused epilogue
+    packer or the separate row packer), with and without the int32 copy;
+    ragged N and K."""
     import torch
     rng = np.random.default_rng(13)
     for bits, (M, N, K), want in ((7, (520, 777, 600), True), (5, (1024, 513, 2000), False),
@@ -581,3 +586,295 @@ def test_plain_chain_matches_fused(am, oracle_mod):
     for p in range(planner.NPLANS):
         outs, stats, _, _ = planner.run_batch(cols, [p] * len(cols), f)
         assert [int(s[2]) for s in stats] == plain, p
+
+
+def test_gen_masks_match_oracle(am, oracle_mod):
+    """Bernoulli and clustered (two-state chain) masks, bit-exact against the
+    oracle's serial restatement, ragged lengths around the 64-bit word and
+    the 4096-bit chain chunk."""
+    for n in (1, 63, 64, 4095, 4096, 4097, 100003, 1 << 20):
+        for s in (0.0, 0.01, 0.5, 0.99, 1.0):
+            say("gen_mask_bernoulli", n, s)
+            got = am.u64(am.gen_mask_bernoulli(11, n, s))[:(n + 63) // 64]
+            assert np.array_equal(got, oracle_mod.gen_mask_bernoulli(11, n, s)), (n, s)
+        for s in (0.01, 0.5, 0.99):
+            for mean in (256.0, 3.0):
+                say("gen_mask_runs", n, s, mean)
+                got = am.u64(am.gen_mask_runs(12, n, s, mean))[:(n + 63) // 64]
+                assert np.array_equal(got, oracle_mod.gen_mask_runs(12, n, s, mean)), (n, s, mean)
+
+
+def test_gen_reference_masks(am, oracle_mod):
+    """The reference driver's generate_mask_uniform and create_bitmask bytes,
+    produced on the GPU, equal the oracle's restatements (pinned to golden
+    bytes of the reference's generator)."""
+    for nbytes in (1, 7, 8, 4099, 1 << 18):
+        for s in (0.01, 0.5, 0.9):
+            say("gen_mask_uniform_ref", nbytes, s)
+            got = am.u64(am.gen_mask_uniform_ref(nbytes, s)).view(np.uint8)[:nbytes]
+            assert np.array_equal(got, oracle_mod.mask_uniform_ref(nbytes, s)), (nbytes, s)
+    for total in (8, 64, 1000, 1 << 20, (1 << 20) + 24):
+        for s, k in ((0.01, 1), (0.5, 8), (0.5, 1), (0.99, 8), (0.3, 3)):
+            say("gen_mask_clusters_ref", total, s, k)
+            got = am.u64(am.gen_mask_clusters_ref(s, k, total)).view(np.uint8)[:total // 8]
+            assert np.array_equal(got, oracle_mod.mask_clusters_ref(s, k, total)), (total, s, k)
+
+
+def _zipf_thresholds(g, s):
+    p = 1.0 / np.arange(1, g + 1) ** s
+    c = np.cumsum(p / p.sum())
+    thr = np.minimum(np.floor(c * 4294967296.0), 4294967295).astype(np.uint64)
+    thr[-1] = 4294967295
+    return thr.astype(np.uint32)
+
+
+def test_gen_from_cdf_and_sorted_noise(am, oracle_mod):
+    """Inverse-CDF draws (the Zipf keys of the group-by and uneven-batch
+    workloads) and sorted-with-noise columns, every format, against the
+    oracle. One threshold table holds exact draw values, so a `<=` for `<`
+    in the search would change those elements."""
+    n = 40009
+    draws = oracle_mod.prng(5, 64)
+    exact = np.unique(np.concatenate([draws[:20], [4294967295]]).astype(np.uint64)).astype(np.uint32)
+    tables = ((_zipf_thresholds(1024, 1.1), 10), (_zipf_thresholds(4096, 1.2), 12), (exact, 5),
+              (np.array([4294967295], np.uint32), 3))
+    for thr, bits in tables:
+        exp = oracle_mod.gen_from_cdf(5, thr, bits, n)
+        for fmt in FORMATS:
+            say("gen_from_cdf", len(thr), bits, fmt)
+            col = am.gen_from_cdf(5, thr, n, bits, fmt)
+            assert np.array_equal(col.host_words(), oracle_mod.pack(fmt, bits, exp)), (len(thr), fmt)
+    for bits, nb in ((3, 1), (11, 5), (20, 14), (31, 25), (40, 32)):
+        exp = oracle_mod.gen_sorted_noise(9, bits, nb, n)
+        for fmt in FORMATS:
+            if not fmt_ok(fmt, bits):
+                continue
+            say("gen_sorted_noise", bits, nb, fmt)
+            col = am.gen_sorted_noise(9, n, bits, fmt, nb)
+            assert np.array_equal(col.host_words(), oracle_mod.pack(fmt, bits, exp)), (bits, fmt)
+
+
+def test_compress_clustered_masks(am, oracle_mod):
+    """Config C2's clustered masks (alternating runs, 1-runs geometric with
+    mean 256) and the reference driver's own mask generators (uniform bytes
+    and 1 or 8 equal clusters, create_bitmask), driving compaction into slab
+    (same and narrower width) and u32 outputs, against the oracle."""
+    n = 3_000_017
+    ws = am.Workspace(n)
+    masks = []
+    for s in (0.01, 0.5, 0.99):
+        masks.append((f"runs{s}", am.gen_mask_runs(21, n, s, 256.0)))
+    masks.append(("ref-uniform0.5", am.gen_mask_uniform_ref(((n + 63) // 64) * 8, 0.5)))
+    masks.append(("ref-1cluster0.01", am.gen_mask_clusters_ref(0.01, 1, n)))
+    masks.append(("ref-8clusters0.5", am.gen_mask_clusters_ref(0.5, 8, n)))
+    for w in (4, 8, 11, 17, 24, 32):
+        vals = oracle_mod.prng(100 + w, n).astype(np.uint64) & np.uint64((1 << w) - 1)
+        for name, m in masks:
+            mw = am.u64(m)[:(n + 63) // 64].copy()
+            for fo, ob in ((am.SLAB, w), (am.U32, w), (am.SLAB, max(1, w - 3))):
+                say("compress-clustered", w, name, fo, ob)
+                _check_compress(am, oracle_mod, vals, w, 1, ob, fo, mask_words=mw, ws=ws)
+    ws.check()
+
+
+def test_config_c3_exact(am, oracle_mod):
+    """Config C3 as specified: 2^28 rows, keys 10-bit Zipf(1.1) over 1024
+    ranks (inverse CDF of the PCG stream, seed 42), values 20-bit uniform
+    (seed 43), both slab-packed; exact u64 SUM + COUNT, key-sorted,
+    minimal-width output; and the chained filter v > 2^19 -> group-by.
+    Expected sums and counts from np.bincount over the host-decoded draws."""
+    n = 1 << 28
+    thr = _zipf_thresholds(1024, 1.1)
+    keys = am.gen_from_cdf(42, thr, n, 10, am.SLAB)
+    vals = am.gen_uniform(43, n, 20, am.SLAB)
+    k = _cdf_rank(thr, oracle_mod.prng(42, n))
+    v = (oracle_mod.prng(43, n) & np.uint32(0xFFFFF)).astype(np.float64)
+    ws = am.Workspace(0)
+    for pred in (None, (am.GT, 1 << 19)):
+        say("config-c3", pred)
+        sel = slice(None) if pred is None else (v > float(1 << 19))
+        kk = k if pred is None else k[sel]
+        vv = v if pred is None else v[sel]
+        cnt = np.bincount(kk, minlength=1024).astype(np.uint64)
+        sm = np.bincount(kk, weights=vv, minlength=1024)  # < 2^53: exact in float64
+        sm = sm.astype(np.uint64)
+        ok, osum, ocnt, res = am.groupby(keys, vals, value_pred=pred, ws=ws)
+        r = am.u64(res)
+        present = np.nonzero(cnt)[0]
+        ng = len(present)
+        assert int(r[0]) == ng and int(r[3]) == int(cnt.sum())
+        assert int(r[1]) == _width(int(sm.max())) and int(r[2]) == _width(int(cnt.max()))
+        assert np.array_equal(oracle_mod.unpack(1, 10, am.u64(ok), ng), present.astype(np.uint64))
+        assert np.array_equal(oracle_mod.unpack(1, int(r[1]), am.u64(osum), ng), sm[present])
+        assert np.array_equal(oracle_mod.unpack(1, int(r[2]), am.u64(ocnt), ng), cnt[present])
+        del ok, osum, ocnt, res
+    ws.check()
+
+
+def _cdf_rank(thr, d):
+    """First b with d < thr[b] (capped at the last bin), i.e. searchsorted
+    'right', through a table over the top 16 bits of d; draws whose
+    65536-wide bucket holds a threshold take the exact search."""
+    top = np.arange(1 << 16, dtype=np.uint64) << np.uint64(16)
+    lo = np.searchsorted(thr, top, side="right").astype(np.int16)
+    hi = np.searchsorted(thr, top + np.uint64(65535), side="right").astype(np.int16)
+    amb_bucket = hi != lo
+    t = d >> np.uint32(16)
+    k = lo[t]
+    amb = np.nonzero(amb_bucket[t])[0]
+    k[amb] = np.searchsorted(thr, d[amb], side="right")
+    return np.minimum(k, len(thr) - 1)
+
+
+def _pack_rows_u7(X):
+    """Row-padded 7-bit slab matrix (9 fields per word) by numpy."""
+    R, C = X.shape
+    cp = -(-C // 9) * 9
+    Y = np.zeros((R, cp), np.uint64)
+    Y[:, :C] = X
+    sh = (np.arange(9, dtype=np.uint64) * np.uint64(7))
+    return (Y.reshape(R, cp // 9, 9) << sh).sum(axis=2, dtype=np.uint64).reshape(-1)
+
+
+def _unpack_rows(words, R, C, bits):
+    eps = 64 // bits
+    wpr = -(-C // eps)
+    W = np.asarray(words[:R * wpr], np.uint64).reshape(R, wpr, 1)
+    sh = np.arange(eps, dtype=np.uint64) * np.uint64(bits)
+    return ((W >> sh) & np.uint64((1 << bits) - 1)).reshape(R, wpr * eps)[:, :C]
+
+
+def test_config_c5_full_size_sampled(am, oracle_mod):
+    """Config C5 at full size: one 8192^3 product of 7-bit operands and a
+    batch of 256 products with M, N, K ~ U[64, 2048]; 4096 sampled entries
+    of the big product and 16 per batch member are checked against int64
+    dot products on the host, in both the int32 and the packed output."""
+    import torch
+    rng = np.random.default_rng(55)
+    M = N = K = 8192
+    A = rng.integers(0, 128, (M, K), dtype=np.uint64)
+    B = rng.integers(0, 128, (K, N), dtype=np.uint64)
+    assert np.array_equal(_pack_rows_u7(A[:3, :100]), _pack_matrix(oracle_mod, A[:3, :100], 7))
+    pa, pb = _pack_rows_u7(A), _pack_rows_u7(B)
+    say("c5 8192^3")
+    cp, ci, ob = am.matmul_packed_batched(
+        7, [(torch.from_numpy(pa.view(np.int64)).cuda(), torch.from_numpy(pb.view(np.int64)).cuda(), M, N, K)],
+        want_i32=True)[0]
+    torch.cuda.synchronize()
+    assert ob == 27
+    i = rng.integers(0, M, 4096)
+    j = rng.integers(0, N, 4096)
+    exp = np.einsum("sk,sk->s", A[i].astype(np.int64), B[:, j].T.astype(np.int64))
+    C32 = ci.view(M, N)
+    got = C32[torch.from_numpy(i).cuda(), torch.from_numpy(j).cuda()].cpu().numpy().astype(np.int64)
+    assert np.array_equal(got, exp)
+    wpr = -(-N // (64 // ob))
+    rows = np.unique(i[:64])
+    P = _unpack_rows(am.u64(cp.view(-1)[:M * wpr].view(M, wpr)[torch.from_numpy(rows).cuda()].reshape(-1)),
+                     len(rows), N, ob)
+    assert np.array_equal(P.astype(np.int64), C32[torch.from_numpy(rows).cuda()].cpu().numpy().astype(np.int64))
+    del cp, ci, C32
+    probs, host = [], []
+    for t in range(256):
+        m, nn, k = (int(x) for x in rng.integers(64, 2049, 3))
+        a = rng.integers(0, 128, (m, k), dtype=np.uint64)
+        b = rng.integers(0, 128, (k, nn), dtype=np.uint64)
+        probs.append((torch.from_numpy(_pack_rows_u7(a).view(np.int64)).cuda(),
+                      torch.from_numpy(_pack_rows_u7(b).view(np.int64)).cuda(), m, nn, k))
+        host.append((a, b))
+    say("c5 batch of 256")
+    outs = am.matmul_packed_batched(7, probs, want_i32=True)
+    torch.cuda.synchronize()
+    for (cp, ci, ob), (a, b), (_, _, m, nn, k) in zip(outs, host, probs):
+        ii = rng.integers(0, m, 16)
+        jj = rng.integers(0, nn, 16)
+        exp = np.einsum("sk,sk->s", a[ii].astype(np.int64), b[:, jj].T.astype(np.int64))
+        Cm = ci.view(m, nn).cpu().numpy().astype(np.int64)
+        assert np.array_equal(Cm[ii, jj], exp), (m, nn, k)
+        assert np.array_equal(_unpack_rows(am.u64(cp), m, nn, ob).astype(np.int64), Cm), (m, nn, k)
+
+
+def _pipeline_cols(am, oracle_mod, rng, ncols, fmt_in):
+    cols, refs = [], []
+    for i in range(ncols):
+        n = int([1, 63, 2048, 4096, 4097, 30011, 200003][i % 7] * (1 + (i // 7) % 2))
+        w = int(rng.integers(3, 32))
+        kind = i % 3
+        if kind == 0:
+            a = rand_vals(rng, n, w)
+        elif kind == 1:
+            g = min(1 << w, 4096)
+            p = 1.0 / np.arange(1, g + 1) ** 1.2
+            a = rng.choice(g, size=n, p=p / p.sum()).astype(np.uint64)
+        else:
+            a = np.minimum((np.arange(n) * ((1 << w) - 1) // max(n, 1)) + rng.integers(0, 8, n), (1 << w) - 1)
+            a = a.astype(np.uint64)
+        b = rand_vals(rng, n, w)
+        c = int(rng.integers(0, 1 << w))
+        fmt = (1, 2)[i % 2]
+        sb = w + 1 if i % 4 else max(1, w - 1)  # some columns too narrow: overflow must be reported
+        cols.append((upload(am, oracle_mod, a, w, fmt_in), upload(am, oracle_mod, b, w, fmt_in), (0, c), fmt, sb))
+        v = (a + b) & np.uint64((1 << (w + 1)) - 1)
+        refs.append((v, a < np.uint64(c), fmt, sb))
+    return cols, refs
+
+
+def _check_pipeline(oracle_mod, refs, outs, st, tag):
+    for i, ((v, sel, fmt, sb), out) in enumerate(zip(refs, outs)):
+        n = len(v)
+        mask = np.packbits(sel, bitorder="little")
+        mask = np.concatenate([mask, np.zeros((-len(mask)) % 8, np.uint8)]).view(np.uint64)
+        k, exp_words, ovf = oracle_mod.compressstore(0, 64, v, n, mask, fmt, sb)
+        total = int(v[sel].sum(dtype=np.uint64))
+        assert [int(x) for x in st[i][:3]] == [k, ovf, total], (tag, i, list(st[i]), k, ovf, total)
+        used = oracle_mod.packed_words(fmt, sb, k)
+        assert np.array_equal(out.host_words()[:used], exp_words[:used]), (tag, i)
+
+
+@pytest.mark.parametrize("fmt_in", [1, 2, 4, 3])
+@pytest.mark.parametrize("variant", [8, 16])
+def test_pipeline_variants_and_formats(am, oracle_mod, fmt_in, variant):
+    """The fused chain for every input decoder (slab, bitstream, u32, tight)
+    and both tile sizes: packed s words, counts, overflow and totals."""
+    import torch
+    rng = np.random.default_rng(100 + 10 * fmt_in + variant)
+    cols, refs = _pipeline_cols(am, oracle_mod, rng, 14, fmt_in)
+    say("pipeline", fmt_in, variant)
+    batch = am.PipelineBatch(cols, variant=variant)
+    batch.run()
+    batch.run()
+    torch.cuda.synchronize()
+    outs, st = batch.results()
+    _check_pipeline(oracle_mod, refs, outs, st, (fmt_in, variant))
+
+
+def test_pipeline_graph_replay(am, oracle_mod):
+    """Config C4's launch form: the batch captured once in a graph and
+    replayed; before every replay the outputs and statistics are cleared
+    eagerly (and other eager work runs), and every replay is checked against
+    the oracle."""
+    import torch
+    rng = np.random.default_rng(77)
+    cols, refs = _pipeline_cols(am, oracle_mod, rng, 21, 1)
+    batch = am.PipelineBatch(cols, variant=16)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        batch.run()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batch.run()
+    for rep in range(4):
+        for o in batch.outs:
+            o.buf.zero_()
+        batch.stats.zero_()
+        junk = am.gen_uniform(rep, 100000, 11, am.SLAB)  # eager work between replays
+        torch.cuda.synchronize()
+        say("pipeline graph replay", rep)
+        g.replay()
+        torch.cuda.synchronize()
+        outs, st = batch.results()
+        _check_pipeline(oracle_mod, refs, outs, st, ("replay", rep))
+        del junk

@@ -295,3 +295,80 @@ def test_matmul_vs_numpy(oracle_mod):
         assert np.array_equal(C, exp)
         # brute force on one element guards against a transposed operand
         assert C[M - 1, 0] == sum(int(A[M - 1, k]) * int(B[k, 0]) for k in range(K))
+
+
+# ------------------------------------------------- this build's generators
+
+def test_gen_mask_bernoulli_pinned(oracle_mod):
+    n = 100003
+    d = oracle_mod.prng(11, n).astype(np.uint64)
+    # s chosen so the threshold equals one draw exactly: `<` leaves that bit clear
+    s = float(d[17]) / 4294967296.0
+    m = oracle_mod.gen_mask_bernoulli(11, n, s)
+    bits = np.unpackbits(m.view(np.uint8), bitorder="little")[:n].astype(bool)
+    assert not bits[17]
+    assert np.array_equal(bits, d < d[17])
+    assert np.array_equal(m, lsb_mask_words(d < d[17]))
+    assert not oracle_mod.gen_mask_bernoulli(11, 1000, 0.0).any()
+    full = oracle_mod.gen_mask_bernoulli(11, 1000, 1.0)
+    assert np.array_equal(full, lsb_mask_words(np.ones(1000, bool)))  # bits past n stay zero
+    for sel in (0.01, 0.5, 0.99):
+        k = np.unpackbits(oracle_mod.gen_mask_bernoulli(3, n, sel).view(np.uint8)).sum()
+        assert abs(k - sel * n) < 5 * np.sqrt(n * sel * (1 - sel)) + 1
+
+
+def _runs(bits):
+    d = np.diff(np.concatenate([[0], bits.astype(np.int8), [0]]))
+    return np.nonzero(d == -1)[0] - np.nonzero(d == 1)[0]
+
+
+def test_gen_mask_runs_pinned(oracle_mod):
+    n = 1 << 22
+    for sel in (0.01, 0.5, 0.99):
+        m = oracle_mod.gen_mask_runs(12, n, sel, 256.0)
+        bits = np.unpackbits(m.view(np.uint8), bitorder="little")[:n]
+        assert abs(bits.mean() - sel) < 0.1 * max(sel, 0.05), sel
+        # chunk c starts from draw c*4097: in state 1 iff that draw < round(s*2^32)
+        chunks = n // 4096
+        starts = np.array([oracle_mod.prng(12, 1, skip=c * 4097)[0] for c in range(0, chunks, 97)], np.uint64)
+        assert np.array_equal(bits[np.arange(0, chunks, 97) * 4096].astype(bool),
+                              starts < np.uint64(int(sel * 4294967296.0 + 0.5)))
+        if sel == 0.5:
+            r = _runs(bits)
+            assert 150 < r.mean() < 300  # geometric, mean 256, cut at chunk ends
+    # mean run 1: a 1-run never lasts past one bit inside a chunk
+    bits = np.unpackbits(oracle_mod.gen_mask_runs(4, 1 << 16, 0.3, 1.0).view(np.uint8), bitorder="little")
+    pairs = bits[:-1] & bits[1:]
+    pairs[np.arange(4095, len(pairs), 4096)] = 0
+    assert not pairs.any()
+
+
+def test_gen_from_cdf_pinned(oracle_mod):
+    n = 50000
+    d = oracle_mod.prng(5, n)
+    thr = np.unique(np.concatenate([d[:40], [4294967295]])).astype(np.uint32)
+    v = oracle_mod.gen_from_cdf(5, thr, 16, n)
+    exp = np.minimum(np.searchsorted(thr, d, side="right"), len(thr) - 1).astype(np.uint64) & np.uint64(0xFFFF)
+    assert np.array_equal(v, exp)
+    # element 0's draw is itself a threshold b: `<` maps it to b + 1
+    b = int(np.searchsorted(thr, d[0]))
+    assert thr[b] == d[0] and v[0] == b + 1
+    # a Zipf(1.1) table over 1024 ranks: rank frequencies follow the weights
+    p = 1.0 / np.arange(1, 1025) ** 1.1
+    c = np.cumsum(p / p.sum())
+    zt = np.minimum(np.floor(c * 4294967296.0), 4294967295).astype(np.uint64)
+    zt[-1] = 4294967295
+    z = oracle_mod.gen_from_cdf(42, zt.astype(np.uint32), 10, 1 << 20)
+    h = np.bincount(z.astype(np.int64), minlength=1024) / (1 << 20)
+    assert np.allclose(h[:8], (p / p.sum())[:8], rtol=0.05)
+
+
+def test_gen_sorted_noise_pinned(oracle_mod):
+    for bits, nb, n in ((3, 1, 1000), (11, 5, 40009), (31, 25, 40009), (40, 32, 10007)):
+        v = oracle_mod.gen_sorted_noise(9, bits, nb, n)
+        m = (1 << bits) - 1
+        i = np.arange(n, dtype=np.float64)
+        base = np.floor((i / float(n)) * float(m)).astype(np.uint64)
+        noise = oracle_mod.prng(9, n).astype(np.uint64) & np.uint64((1 << nb) - 1)
+        assert np.array_equal(v, np.minimum(base + noise, np.uint64(m)))
+        assert (np.diff(base.astype(np.int64)) >= 0).all() and (v <= np.uint64(m)).all()
