@@ -40,6 +40,8 @@ def parse():
                    help="comma-separated legs to run after the headline (sweep, groupby, matmul, batched)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--reps", type=int, default=10, help="launches per sweep point")
+    p.add_argument("--csv", default=None, metavar="DIR",
+                   help="also write every timed launch of the sweep in the reference's CSV schema under DIR")
     p.add_argument("--profile-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (reference point for HBM counters)")
     return p.parse_args()
@@ -54,6 +56,7 @@ class Timer:
 
     def __init__(self, torch):
         self.torch = torch
+        self.last = []
 
     def median_ms(self, fn, reps, warm=2):
         torch = self.torch
@@ -65,7 +68,8 @@ class Timer:
             fn()
             b.record()
         torch.cuda.synchronize()
-        return statistics.median(a.elapsed_time(b) for a, b in evs)
+        self.last = [a.elapsed_time(b) for a, b in evs]
+        return statistics.median(self.last)
 
 
 def read_traffic(name):
@@ -125,6 +129,19 @@ def cpu_baseline(am, torch, col, pred_c):
                          "ms": round(res[mt_threads][1] * 1e3, 2)}}
 
 
+def needed_line_bytes(torch, mask, n, eps):
+    """Bytes of the 128-byte input lines (16 slab words = 16*eps elements)
+    that hold at least one selected element: the input a compaction that
+    skips empty regions has to read."""
+    words = mask[:(n + 63) // 64]
+    bits = ((words.view(-1, 1) >> torch.arange(64, device=words.device)) & 1).view(-1)[:n].bool()
+    L = 16 * eps
+    pad = (-n) % L
+    if pad:
+        bits = torch.cat([bits, torch.zeros(pad, dtype=torch.bool, device=bits.device)])
+    return int(bits.view(-1, L).any(dim=1).sum().item()) * 128
+
+
 def sweep(am, torch, timer, reps):
     """Per-op achieved GB/s at N = 2^28 (algorithmic bytes)."""
     ops = []
@@ -132,18 +149,23 @@ def sweep(am, torch, timer, reps):
     ws = am.Workspace(n)
     stats = am.device_words(2)
     mask_words = (n + 63) // 64
+
+    def rec(d, ms, byt):
+        d.update({"n": n, "ms": ms, "GBs": gbytes(byt, ms), "times": list(timer.last)})
+        ops.append(d)
+
     for w in WIDTHS:
         print(f"sweep w={w}", file=sys.stderr, flush=True)
         a = am.gen_uniform(42, n, w, am.SLAB)
         b = am.gen_uniform(43, n, w, am.SLAB)
         in_b = a.words * 8
+        eps = 64 // w
         torch.cuda.synchronize()
 
         # filter -> mask
         mo = am.device_words(mask_words)
         ms = timer.median_ms(lambda: am.filter(a, am.LT, 1 << (w - 1), mask_out=mo), reps)
-        ops.append({"op": "filter", "w": w, "in": "slab", "out": "mask", "ms": ms,
-                    "GBs": gbytes(in_b + mask_words * 8, ms)})
+        rec({"op": "filter", "w": w, "in": "slab", "out": "mask"}, ms, in_b + mask_words * 8)
 
         # fused filter + compressstore at 1/50/99 % selectivity, three output formats
         for frac in (0.01, 0.5, 0.99):
@@ -156,23 +178,34 @@ def sweep(am, torch, timer, reps):
                 k = int(am.u64(stats)[0])
                 ms = timer.median_ms(lambda: am.compressstore(a, ob, fo, pred=(am.LT, c), ws=ws, out=out, stats=stats),
                                      reps)
-                byt = in_b + am.packed_words(fo, ob, k) * 8
-                ops.append({"op": "filter+compressstore", "w": w, "sel": frac, "in": "slab", "out": label, "ob": ob,
-                            "ms": ms, "GBs": gbytes(byt, ms)})
+                rec({"op": "filter+compressstore", "w": w, "sel": frac, "in": "slab", "out": label, "ob": ob},
+                    ms, in_b + am.packed_words(fo, ob, k) * 8)
                 del out
 
-        # mask-driven compressstore (reference semantics: slab -> slab same w), uniform and clustered masks
-        for frac in (0.01, 0.5, 0.99):
-            for shape in ("uniform", "clustered"):
-                m = (am.gen_mask_bernoulli(7, n, frac) if shape == "uniform" else am.gen_mask_runs(7, n, frac, 256.0))
+        # mask-driven compressstore (reference semantics: slab -> slab same w):
+        # uniform Bernoulli masks, clustered masks (alternating geometric
+        # runs, mean 256 ones), and the reference driver's own create_bitmask
+        # masks with 1 and 8 equal clusters
+        shapes = (("uniform", (0.01, 0.5, 0.99)), ("clustered", (0.01, 0.5, 0.99)),
+                  ("ref_1_cluster", (0.01, 0.5)), ("ref_8_clusters", (0.01, 0.5)))
+        for shape, fracs in shapes:
+            for frac in fracs:
+                if shape == "uniform":
+                    m = am.gen_mask_bernoulli(7, n, frac)
+                elif shape == "clustered":
+                    m = am.gen_mask_runs(7, n, frac, 256.0)
+                else:
+                    m = am.gen_mask_clusters_ref(frac, 1 if shape == "ref_1_cluster" else 8, n)
                 out = am.Column(n, w, am.SLAB)
                 am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats)
                 torch.cuda.synchronize()
                 k = int(am.u64(stats)[0])
                 ms = timer.median_ms(lambda: am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats), reps)
-                byt = in_b + mask_words * 8 + am.packed_words(am.SLAB, w, k) * 8
-                ops.append({"op": "compressstore(mask)", "w": w, "sel": frac, "mask": shape, "in": "slab",
-                            "out": "slab_same_w", "ms": ms, "GBs": gbytes(byt, ms)})
+                out_b = am.packed_words(am.SLAB, w, k) * 8
+                need = needed_line_bytes(torch, m, n, eps)
+                rec({"op": "compressstore(mask)", "w": w, "sel": frac, "mask": shape, "in": "slab",
+                     "out": "slab_same_w", "needed_GBs": round(gbytes(need + mask_words * 8 + out_b, ms), 1)},
+                    ms, in_b + mask_words * 8 + out_b)
                 del out, m
 
         # binary add (w+1) and and (w)
@@ -180,14 +213,13 @@ def sweep(am, torch, timer, reps):
             ob = am.outbits(op, w)
             out = am.Column(n, ob, am.SLAB)
             ms = timer.median_ms(lambda: am.binary(op, a, b, out=out), reps)
-            ops.append({"op": name, "w": w, "in": "slab", "out": f"slab w={ob}", "ms": ms,
-                        "GBs": gbytes(2 * in_b + out.words * 8, ms)})
+            rec({"op": name, "w": w, "in": "slab", "out": f"slab w={ob}"}, ms, 2 * in_b + out.words * 8)
             del out
 
         # u64 sum
         s = am.device_words(1)
         ms = timer.median_ms(lambda: am.sum(a, ws=ws, out=s), reps)
-        ops.append({"op": "sum", "w": w, "in": "slab", "ms": ms, "GBs": gbytes(in_b + 8, ms)})
+        rec({"op": "sum", "w": w, "in": "slab"}, ms, in_b + 8)
 
         # bitstream variants at w = 11
         if w == 11:
@@ -195,14 +227,12 @@ def sweep(am, torch, timer, reps):
             bb = am.convert(b, w, am.BITSTREAM)
             ib = ab.words * 8
             ms = timer.median_ms(lambda: am.filter(ab, am.LT, 1 << (w - 1), mask_out=mo), reps)
-            ops.append({"op": "filter", "w": w, "in": "bitstream", "out": "mask", "ms": ms,
-                        "GBs": gbytes(ib + mask_words * 8, ms)})
+            rec({"op": "filter", "w": w, "in": "bitstream", "out": "mask"}, ms, ib + mask_words * 8)
             out = am.Column(n, w + 1, am.BITSTREAM)
             ms = timer.median_ms(lambda: am.binary(am.ADD, ab, bb, out=out), reps)
-            ops.append({"op": "add", "w": w, "in": "bitstream", "out": "bitstream w=12", "ms": ms,
-                        "GBs": gbytes(2 * ib + out.words * 8, ms)})
+            rec({"op": "add", "w": w, "in": "bitstream", "out": "bitstream w=12"}, ms, 2 * ib + out.words * 8)
             ms = timer.median_ms(lambda: am.sum(ab, ws=ws, out=s), reps)
-            ops.append({"op": "sum", "w": w, "in": "bitstream", "ms": ms, "GBs": gbytes(ib + 8, ms)})
+            rec({"op": "sum", "w": w, "in": "bitstream"}, ms, ib + 8)
             del ab, bb, out
         del a, b
         torch.cuda.empty_cache()
@@ -211,6 +241,59 @@ def sweep(am, torch, timer, reps):
         o["GBs"] = round(o["GBs"], 1)
         o["pct_peak"] = round(100.0 * o["GBs"] / PEAK_GBS, 1)
     return ops
+
+
+# The reference's result schema (src/main.cu:300, src/common_code.cuh:206-213):
+# one line per timed launch, throughput = used_data_size / time in GiB/s,
+# where used_data_size counts 8 bytes per element per input array
+# ("uncompressed-equivalent"), plus the mask bytes for compaction. Files are
+# split by workload as in the reference's data/ tree (one sweep file, one
+# file per mask distribution and selectivity under comp/).
+CSV_HEADER = "approach;element_count;bits;block_count;thread_count;time_ms;throughput;run"
+
+
+def csv_place(o):
+    """(file, approach, used_data_size) of a sweep record, or None."""
+    n, op = o.get("n"), o["op"]
+    if n is None:
+        return None
+    mask_b = (n + 7) // 8
+    fmt = "nogaps" if o.get("in") == "bitstream" else "slabs"
+    if op == "filter":
+        return ("sweep", f"filter_{fmt}", 8 * n)
+    if op == "add":
+        return ("sweep", f"binary_op_{fmt}", 16 * n)
+    if op == "and":
+        return ("sweep", f"binary_op_{fmt}_and", 16 * n)
+    if op == "sum":
+        return ("sweep", f"hop_{fmt}", 8 * n)
+    if op.startswith("groupby") or op.endswith("groupby"):
+        return ("sweep", "groupby_sm_array" if op.startswith("groupby") else "filter_groupby_sm_array", 16 * n)
+    short = {"uniform": "uni", "clustered": "runs", "ref_1_cluster": "sc", "ref_8_clusters": "mc"}
+    if op == "compressstore(mask)":
+        return (f"comp/{short[o['mask']]}_{o['sel']}", "compressstore_slabs", 8 * n + mask_b)
+    if op == "filter+compressstore":
+        return (f"comp/pred_{o['sel']}", f"filter_compressstore_slabs_to_{o['out']}", 8 * n + mask_b)
+    return None
+
+
+def write_csv(ops, outdir):
+    files = {}
+    for o in ops:
+        place = csv_place(o)
+        if place is None or "times" not in o:
+            continue
+        f, approach, used = place
+        lines = files.setdefault(f, [CSV_HEADER])
+        for run, t in enumerate(o["times"]):
+            thr = (used / (t / 1000.0)) / (1 << 30)
+            lines.append(f"{approach};{o['n']};{o['w'] if 'w' in o else o.get('key_bits', 10)};-1;-1;{t:f};{thr:f};{run}")
+    for f, lines in files.items():
+        path = os.path.join(outdir, f)
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+    return sorted(files)
 
 
 def groupby_leg(am, torch, timer, reps):
@@ -228,14 +311,15 @@ def groupby_leg(am, torch, timer, reps):
     out = []
     for label, pred in (("groupby sum+count", None), ("filter(v>2^19)->groupby", (am.GT, 1 << 19))):
         ms = timer.median_ms(lambda: am.groupby(keys, vals, value_pred=pred, ws=ws), reps)
+        times = list(timer.last)
         _, _, _, res = am.groupby(keys, vals, value_pred=pred, ws=ws)
         torch.cuda.synchronize()
         r = am.u64(res)
         g = int(r[0])
         byt = keys.words * 8 + vals.words * 8 + g * 8 * 3
-        out.append({"op": label, "rows": n, "groups": g, "sum_bits": int(r[1]), "count_bits": int(r[2]),
-                    "ms": round(ms, 4), "GBs": round(gbytes(byt, ms), 1),
-                    "pct_peak": round(100 * gbytes(byt, ms) / PEAK_GBS, 1)})
+        out.append({"op": label, "rows": n, "n": n, "key_bits": 10, "groups": g, "sum_bits": int(r[1]),
+                    "count_bits": int(r[2]), "ms": round(ms, 4), "GBs": round(gbytes(byt, ms), 1),
+                    "pct_peak": round(100 * gbytes(byt, ms) / PEAK_GBS, 1), "times": times})
     return out
 
 
@@ -306,6 +390,11 @@ def batched_leg(am, torch, reps):
     fr = feat.cpu().numpy()
     outs, stats, t_batch, redo = planner.run_batch(cols, list(plans), fr, reps=max(3, reps // 2))
     ok = [int(s[2]) for s in stats] == plain_totals
+    # algorithmic bytes of the fused chain: both inputs read once, the
+    # compacted sums written at each column's planned format and width
+    byt = 0
+    for (a, b, _), o, st in zip(cols, outs, stats):
+        byt += a.words * 8 + b.words * 8 + am.packed_words(o.fmt, o.bits, int(st[0])) * 8
     # graph replay of the batched launches (same plans, no fallback needed now)
     groups = {}
     for i, p in enumerate(plans):
@@ -344,7 +433,9 @@ def batched_leg(am, torch, reps):
                  for j, i in enumerate(idx))
     return {"columns": len(cols), "rows": rows, "plain_s": round(t_plain, 5), "batched_s": round(t_batch, 5),
             "graph_s": round(t_graph, 5), "plan_s": round(t_plan, 5), "planner": kind,
+            "bytes": byt, "GBs": round(byt / t_batch / 1e9, 1), "pct_peak": round(100 * byt / t_batch / 1e9 / PEAK_GBS, 1),
             "speedup_batched": round(t_plain / t_batch, 2), "speedup_graph": round(t_plain / t_graph, 2),
+            "speedup_incl_planning": round(t_plain / (t_batch + t_plan), 2),
             "fallback_reruns": redo, "totals_match_plain": bool(ok and res_ok)}
 
 
@@ -428,6 +519,11 @@ def main():
     del src, dst
     print("cpu baseline", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(am, torch, col, c)
+
+    if args.csv:
+        extra["csv_files"] = write_csv(ops, args.csv)
+    for o in ops:
+        o.pop("times", None)
 
     line = {
         "metric": "HBM GB/s (% of peak) per op; batched+learned plan vs plain plan speedup",
