@@ -296,8 +296,27 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
     }
     {
         const uint32_t wl0 = wave * 64 * R, wend = wl0 + 64 * R;
+        // Mask input: chunk r of this wave (64 words, 64*eps elements) is
+        // exactly mask words [r*eps, (r+1)*eps) of the wave's R*eps, so the
+        // wave reads its mask words first and loads only the input words of
+        // chunks that select something (empty regions are never read: the
+        // reference skips empty 1024-element chunks the same way,
+        // src/kernels/compressstore_slabs.cuh:75-85).
+        uint32_t live = 0xFFFFFFFFu;
+        if (!PRED) {
+            const uint32_t nwm = R * eps; // <= 64
+            const uint64_t mi = mw0 + (uint64_t)wave * nwm + lane;
+            uint64_t m = 0;
+            if (lane < nwm && mi < nmw) m = a.mask_in[mi];
+            if (lane < nwm) s_mask[wave * nwm + lane] = m;
+            const uint64_t nz = ballot(m != 0);
+            live = 0;
+            for (uint32_t r = 0; r < R; r++)
+                if ((nz >> (r * eps)) & lowmask(eps)) live |= 1u << r;
+        }
         if (a.lds16) {
             for (uint32_t j = 0; j < R; j += 2) { // 128 words per instruction, 16 bytes per lane
+                if (!((live >> j) & 3u)) continue; // (wave-uniform)
                 const uint32_t wl = wl0 + j * 64 + 2 * lane;
                 const uint64_t gw = w0 + wl;
                 if (wl < wend) {
@@ -309,15 +328,11 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         else {
             const uint32_t* src = (const uint32_t*)a.in.p;
             for (uint32_t j = 0; j < 2 * R; j++) { // 32 words per instruction, 4 bytes per lane
+                if (!((live >> (j >> 1)) & 1u)) continue; // (wave-uniform)
                 const uint32_t dl = 2 * wl0 + j * 64 + lane;
                 if (2 * w0 + dl < 2 * nw)
                     __builtin_amdgcn_global_load_lds(src + 2 * w0 + dl, (lds_ptr)((uint32_t*)s_x + 2 * wl0 + j * 64), 4, 0, LD_NT);
             }
-        }
-        if (!PRED) {
-            uint64_t m = 0;
-            if (tid < NMW && mw0 + tid < nmw) m = a.mask_in[mw0 + tid];
-            if (tid < NMW) s_mask[tid] = m;
         }
         __builtin_amdgcn_s_waitcnt(0); // this wave's global->LDS copies
     }
@@ -478,6 +493,14 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         L.publish_agg(t, own, lane);
         if ((t & 63) == 63) L.publish_group_early(t, own, lane);
     }
+    if (ctile == 0 && !last && (t & 63) != 63) {
+        // A tile that selects nothing writes nothing and needs no prefix:
+        // with its (empty) aggregate out, successors walk past it. (A
+        // group's last tile still walks: later walks stop at its inclusive
+        // value; the last tile writes the statistics and the tail.)
+        if (lead) make_lookback<F>(a, s_epoch).clear_slice(a.T, t * 64 + lane, (uint64_t)a.T * 64);
+        return;
+    }
     // (every LDS operation so far has completed; saying so keeps the compiler
     // from re-waiting on them inside the staging loops below, where each such
     // wait would also drain the previous value store)
@@ -627,11 +650,14 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
     Fetch<IK, WIDE> f[CE_J];
     uint64_t mk = 0;
     if (wrem) {
-#pragma unroll
-        for (uint32_t j = 0; j < CE_J; j++) f[j].load(rd, idx(j));
-        if (!PRED) {
+        if (!PRED) { // mask input: 64-element chunks with an empty mask word are not read
             const uint64_t w = (wb >> 6) + lane;
             mk = (lane < CE_J && w < nmw) ? a.mask_in[w] : 0;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < CE_J; j++) {
+            f[j].raw = 0;
+            if (PRED || readlane64(mk, j) != 0) f[j].load(rd, idx(j));
         }
     }
     uint64_t mine = 0;
@@ -664,6 +690,10 @@ __global__ __launch_bounds__(CS_THREADS) void k_compress_elems(CompressArgs a)
         wpre += w < wave ? c : 0;
         ctile += c;
         otile += s_wovf[w];
+    }
+    if (ctile == 0 && t != a.T - 1 && (t & 63) != 63) { // selects nothing: publish the empty aggregate, done
+        if (wave == 0) L.publish_agg(t, Agg{0, 0, 0, 0}, lane);
+        return;
     }
     if (OK == K_STRIDED && wave == 0) tile_lookback<F>(a, L, t, ctile, otile, 0, lane, &s_P, &s_carry);
     uint32_t run = wpre;
