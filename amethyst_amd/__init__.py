@@ -106,6 +106,7 @@ def lib():
             "am_binary": ([u32, colp, colp, colp, vp], i32),
             "am_sum": ([colp, vp, vp, u64, vp], i32),
             "am_groupby": ([colp, colp, predp, u32, vp, vp, vp, vp, vp, u64, vp], i32),
+            "am_groupby_workspace_bytes": ([u64, u32, u32], u64),
             "am_pipeline_workspace_bytes": ([vp, u32], u64),
             "am_pipeline_prepare": ([vp, u32, u32, vp, u64, vp], i32),
             "am_pipeline_run": ([u32, u32, u32, u32, vp, u64, vp], i32),
@@ -213,6 +214,14 @@ class Workspace:
     def __init__(self, max_elements, device="cuda"):
         self.bytes = workspace_bytes(max_elements)
         self.buf = device_words((self.bytes + 7) // 8, device)  # torch.zeros: already zero
+
+    @classmethod
+    def for_groupby(cls, n, key_bits, value_bits, device="cuda"):
+        """A workspace large enough for am_groupby over n rows at these widths."""
+        ws = cls.__new__(cls)
+        ws.bytes = max(workspace_bytes(0), int(lib().am_groupby_workspace_bytes(n, key_bits, value_bits)))
+        ws.buf = device_words((ws.bytes + 7) // 8, device)
+        return ws
 
     def ptr(self):
         return ctypes.c_void_p(self.buf.data_ptr())
@@ -371,7 +380,7 @@ def groupby(keys, values, mode=GB_EXACT, value_pred=None, want_counts=True, ws=N
     cap = max(cap, g)
     ok, osum, ocnt = device_words(cap), device_words(cap), device_words(cap) if want_counts else None
     res = device_words(4)
-    ws = ws if ws is not None else Workspace(0)
+    ws = ws if ws is not None else Workspace.for_groupby(keys.n, keys.bits, values.bits)
     ck, cv = keys.c(), values.c()
     pp = ctypes.byref(am_predicate(value_pred[0], 0, value_pred[1])) if value_pred is not None else None
     _check(lib().am_groupby(ctypes.byref(ck), ctypes.byref(cv), pp, mode, ctypes.c_void_p(ok.data_ptr()),

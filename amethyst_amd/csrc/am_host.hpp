@@ -10,6 +10,13 @@
 // descriptor bytes of the largest compaction launch over n elements (k_compress.hip)
 uint64_t am_compress_desc_bytes(uint64_t n);
 
+namespace am { struct Col; }
+// group-by over keys of 13..24 bits (k_groupby_wide.hip): workspace bytes and launch
+uint64_t am_groupby_wide_bytes(uint64_t n, uint32_t kbits, uint32_t vbits);
+int am_groupby_wide(const am::Col& ck, const am::Col& cv, const am_predicate* value_pred, uint32_t mode,
+                    uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts, uint64_t* result_dev, void* ws,
+                    uint64_t ws_bytes, hipStream_t s);
+
 namespace am {
 
 inline uint32_t tight_bytes(uint32_t bits)

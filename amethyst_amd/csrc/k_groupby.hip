@@ -325,6 +325,12 @@ __global__ __launch_bounds__(GB_THREADS) void k_groupby_writeout(GroupArgs a)
 
 using namespace am;
 
+extern "C" uint64_t am_groupby_workspace_bytes(uint64_t n, uint32_t key_bits, uint32_t value_bits)
+{
+    if (key_bits < 1 || key_bits > 24 || value_bits < 1 || value_bits > 32) return 0;
+    return key_bits <= 12 ? WS_FIXED : am_groupby_wide_bytes(n, key_bits, value_bits);
+}
+
 extern "C" int am_groupby(const am_column* keys, const am_column* values, const am_predicate* value_pred, uint32_t mode,
                           uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts, uint64_t* result_dev, void* ws,
                           uint64_t ws_bytes, void* stream)
@@ -335,8 +341,11 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     if ((st = make_col(values, &cv))) return st;
     if (ck.n != cv.n || !out_keys || !out_sums || !result_dev) return AM_E_INVALID;
     if (mode > AM_GB_REF || (value_pred && value_pred->op > AM_PRED_NE)) return AM_E_INVALID;
-    if (ck.bits > 12 || cv.bits > 32) return AM_E_UNSUPPORTED_CONFIG;
+    if (ck.bits > 24 || cv.bits > 32) return AM_E_UNSUPPORTED_CONFIG;
     if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
+    if (ck.bits > 12) // partitioned path
+        return am_groupby_wide(ck, cv, value_pred, mode, out_keys, out_sums, out_counts, result_dev, ws, ws_bytes,
+                               (hipStream_t)stream);
     GroupArgs a{};
     a.keys = ck;
     a.vals = cv;

@@ -267,6 +267,8 @@ def csv_place(o):
         return ("sweep", f"binary_op_{fmt}_and", 16 * n)
     if op == "sum":
         return ("sweep", f"hop_{fmt}", 8 * n)
+    if "keys" in op:  # wide keys: the reference's sort-based group-by
+        return ("sweep", "groupby_sort", 16 * n)
     if op.startswith("groupby") or op.endswith("groupby"):
         return ("sweep", "groupby_sm_array" if op.startswith("groupby") else "filter_groupby_sm_array", 16 * n)
     short = {"uniform": "uni", "clustered": "runs", "ref_1_cluster": "sc", "ref_8_clusters": "mc"}
@@ -320,6 +322,31 @@ def groupby_leg(am, torch, timer, reps):
         out.append({"op": label, "rows": n, "n": n, "key_bits": 10, "groups": g, "sum_bits": int(r[1]),
                     "count_bits": int(r[2]), "ms": round(ms, 4), "GBs": round(gbytes(byt, ms), 1),
                     "pct_peak": round(100 * gbytes(byt, ms) / PEAK_GBS, 1), "times": times})
+    del keys
+    # wide keys (partitioned path): 24-bit uniform keys, 16-bit Zipf(1.1) ranks
+    p16 = 1.0 / np.arange(1, 65537) ** 1.1
+    c16 = np.cumsum(p16 / p16.sum())
+    thr16 = np.minimum(np.floor(c16 * 4294967296.0), 4294967295).astype(np.uint64)
+    thr16[-1] = 4294967295
+    for label, kb, kcol in (("groupby 24-bit uniform keys", 24, lambda: am.gen_uniform(44, n, 24, am.SLAB)),
+                            ("groupby 16-bit Zipf(1.1) keys", 16,
+                             lambda: am.gen_from_cdf(45, thr16.astype(np.uint32), n, 16, am.SLAB))):
+        keys = kcol()
+        wsw = am.Workspace.for_groupby(n, kb, 20)
+        ms = timer.median_ms(lambda: am.groupby(keys, vals, ws=wsw), reps)
+        times = list(timer.last)
+        _, _, _, res = am.groupby(keys, vals, ws=wsw)
+        torch.cuda.synchronize()
+        r = am.u64(res)
+        g = int(r[0])
+        outb = (am.packed_words(am.SLAB, kb, g) + am.packed_words(am.SLAB, int(r[1]), g)
+                + am.packed_words(am.SLAB, int(r[2]), g)) * 8
+        byt = keys.words * 8 + vals.words * 8 + outb
+        out.append({"op": label, "rows": n, "n": n, "key_bits": kb, "groups": g, "sum_bits": int(r[1]),
+                    "count_bits": int(r[2]), "ms": round(ms, 4), "GBs": round(gbytes(byt, ms), 1),
+                    "pct_peak": round(100 * gbytes(byt, ms) / PEAK_GBS, 1), "times": times})
+        del keys, wsw
+        torch.cuda.empty_cache()
     return out
 
 

@@ -271,7 +271,13 @@ int am_binary(uint32_t op, const am_column* a, const am_column* b, am_column* ou
  * with HOP_INIT = 0, HOP_OP = + (src/config.cuh:63-71). */
 int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t ws_bytes, void* stream);
 
-/* Group-by over keys (bits <= 12) with values (bits <= 32). Rows whose value
+/* Workspace bytes am_groupby needs for n rows at these widths (0 for an
+ * unsupported width): the fixed workspace for keys of <= 12 bits; keys of
+ * 13..24 bits also need room for the partitioned rows (4 or 8 bytes each)
+ * and a dense table of 2^key_bits sums and counts. */
+uint64_t am_groupby_workspace_bytes(uint64_t n, uint32_t key_bits, uint32_t value_bits);
+
+/* Group-by over keys (bits <= 24) with values (bits <= 32). Rows whose value
  * fails `value_pred` are skipped when value_pred != NULL (the chained
  * filter -> group-by). Output is in ascending key order:
  *   out_keys   : present keys, slab-packed at keys->bits
@@ -284,7 +290,13 @@ int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t ws_bytes, 
  * (>= 1); in AM_GB_REF sums are mod 2^values->bits and stored at that width.
  * Replaces kernel_groupby_sm_array_gather / _writeout
  * (src/kernels/groupby_sm_array.cuh:4-81), whose output order is
- * nondeterministic; ours is key-sorted. */
+ * nondeterministic; ours is key-sorted. Keys of 13..24 bits (n < 2^32) take
+ * a partitioned path: rows are split by the key's high bits and each
+ * partition is aggregated in an LDS table over its low bits -- the
+ * counterpart of the reference's sort-based group-by (cub::DeviceRadixSort::
+ * SortPairs + cub::DeviceReduce::ReduceByKey, src/benchs/
+ * bench_groupby_sort_cub.cu:50-63), with the same outputs as the small-key
+ * path. */
 int am_groupby(const am_column* keys, const am_column* values, const am_predicate* value_pred,
                uint32_t mode, uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts,
                uint64_t* result_dev, void* ws, uint64_t ws_bytes, void* stream);

@@ -90,11 +90,18 @@ def test_status_strings_and_errors(L):
     assert L.am_convert(ctypes.byref(col65), ctypes.byref(ok), None) == 1
     u33 = am.am_column(None, 0, 33, am.U32)
     assert L.am_filter(ctypes.byref(u33), am.am_predicate(0, 0, 0), None, None) == 1
-    keys13 = am.am_column(ctypes.c_void_p(8), 1, 13, am.SLAB)
+    keys25 = am.am_column(ctypes.c_void_p(8), 1, 25, am.SLAB)
+    keys20 = am.am_column(ctypes.c_void_p(8), 1, 20, am.SLAB)
     vals = am.am_column(ctypes.c_void_p(8), 1, 20, am.SLAB)
     dummy = ctypes.c_void_p(8)
-    # keys wider than 12 bits: unsupported (the reference caps shared-memory group-by at 12 bits)
-    assert L.am_groupby(ctypes.byref(keys13), ctypes.byref(vals), None, 0, dummy, dummy, dummy, dummy, dummy,
+    # keys wider than 24 bits: unsupported
+    assert L.am_groupby(ctypes.byref(keys25), ctypes.byref(vals), None, 0, dummy, dummy, dummy, dummy, dummy,
                         1 << 30, None) == 2
+    # 13..24-bit keys need the partitioned path's workspace (checked before any launch)
+    need = L.am_groupby_workspace_bytes(1, 20, 20)
+    assert need > L.am_groupby_workspace_bytes(1, 12, 20) > 0
+    assert L.am_groupby(ctypes.byref(keys20), ctypes.byref(vals), None, 0, dummy, dummy, dummy, dummy, dummy,
+                        need - 1, None) == 4
+    assert L.am_groupby_workspace_bytes(1, 25, 20) == 0 and L.am_groupby_workspace_bytes(1, 10, 33) == 0
     # missing workspace
     assert L.am_sum(ctypes.byref(vals), dummy, None, 0, None) == 4

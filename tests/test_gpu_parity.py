@@ -878,3 +878,55 @@ def test_pipeline_graph_replay(am, oracle_mod):
         outs, st = batch.results()
         _check_pipeline(oracle_mod, refs, outs, st, ("replay", rep))
         del junk
+
+
+def _check_groupby(am, oracle_mod, keys, vals, kbits, vbits, kf, vf, mode, pred, ws, tag):
+    n = len(keys)
+    kc = upload(am, oracle_mod, keys, kbits, kf)
+    vc = upload(am, oracle_mod, vals, vbits, vf)
+    ok, osum, ocnt, res = am.groupby(kc, vc, mode=mode, value_pred=pred, ws=ws)
+    r = am.u64(res)
+    sums, counts, rows = oracle_mod.groupby(kf, kbits, oracle_mod.pack(kf, kbits, keys), vf, vbits,
+                                            oracle_mod.pack(vf, vbits, vals), n, pred)
+    present = np.nonzero(counts)[0]
+    ng = len(present)
+    exp_s = sums[present]
+    if mode == 1:
+        exp_s = exp_s & np.uint64((1 << vbits) - 1)
+    sb = vbits if mode == 1 else _width(exp_s.max() if ng else 0)
+    cb = _width(counts.max())
+    assert list(map(int, r)) == [ng, sb, cb, rows], (tag, list(map(int, r)), ng, sb, cb, rows)
+    assert np.array_equal(oracle_mod.unpack(1, kbits, am.u64(ok), ng), present.astype(np.uint64)), tag
+    assert np.array_equal(oracle_mod.unpack(1, sb, am.u64(osum), ng), exp_s), tag
+    assert np.array_equal(oracle_mod.unpack(1, cb, am.u64(ocnt), ng), counts[present]), tag
+    for buf, bits in ((ok, kbits), (osum, sb), (ocnt, cb)):  # dead and padding bits zero
+        wds = oracle_mod.packed_words(1, bits, ng)
+        assert np.array_equal(am.u64(buf)[:wds], oracle_mod.pack(1, bits, oracle_mod.unpack(1, bits, am.u64(buf), ng)))
+
+
+def test_groupby_wide_keys(am, oracle_mod):
+    """Keys of 13..24 bits (partitioned path) against the oracle: uniform keys
+    (partitions of one aggregation chunk, stored directly) and Zipf(1.1)
+    ranks (a few partitions of many chunks, merged with atomics), slab /
+    bitstream / container inputs, both modes, with and without the chained
+    value predicate, plus an empty selection."""
+    rng = np.random.default_rng(31)
+    for kbits, vbits, n, dist in ((13, 20, 100003, "zipf"), (16, 20, 2000003, "zipf"), (16, 32, 300007, "uniform"),
+                                  (20, 1, 50000, "uniform"), (24, 20, 3000017, "uniform"), (24, 32, 1000003, "zipf"),
+                                  (18, 11, 1, "uniform")):
+        g = 1 << kbits
+        if dist == "zipf":
+            p = 1.0 / np.arange(1, g + 1) ** 1.1
+            keys = rng.choice(g, size=n, p=p / p.sum()).astype(np.uint64)
+        else:
+            keys = rng.integers(0, g, size=n, dtype=np.uint64)
+        vals = rand_vals(rng, n, vbits)
+        ws = am.Workspace.for_groupby(n, kbits, vbits)
+        for kf, vf in ((1, 1), (2, 2), (4, 3)):
+            for mode in (0, 1):
+                for pred in (None, (2, (1 << (vbits - 1)) - 1), (0, 0)):
+                    if pred is not None and (mode, kf) != (0, 1):
+                        continue
+                    say("groupby-wide", kbits, vbits, n, dist, kf, vf, mode, pred)
+                    _check_groupby(am, oracle_mod, keys, vals, kbits, vbits, kf, vf, mode, pred, ws,
+                                   (kbits, vbits, n, kf, vf, mode, pred))
