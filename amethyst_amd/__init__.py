@@ -17,6 +17,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamethyst.so")
+TUNE_TABLE = os.path.join(_HERE, "tune_table.txt")
 _LIB = None
 
 # formats (include/amethyst.h)
@@ -28,6 +29,9 @@ LT, LE, GT, GE, EQ, NE = range(6)
 (ADD, MUL, MOD, DIV, MATH, BOOL_SIMPLE, BOOL_COMPLEX, AND, OR, XOR, SUB) = range(11)
 # group-by modes
 GB_EXACT, GB_REF = 0, 1
+# launch-shape knobs (am_knob)
+(KNOB_FILTER_WORDS, KNOB_COMPRESS_CHUNKS, KNOB_BINARY_WORDS, KNOB_SUM_BLOCKS, KNOB_GROUPBY_BLOCKS,
+ KNOB_PIPELINE_TILE) = range(6)
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "unsupported configuration", 3: "HIP runtime error",
           4: "workspace missing or too small", 5: "output capacity too small",
@@ -117,12 +121,21 @@ def lib():
             "am_matmul_workspace_bytes": ([vp, vp, vp, u32], u64),
             "am_matmul_packed": ([u32, vp, vp, u32, u32, u32, u32, vp, vp, vp, u64, vp], i32),
             "am_matmul_packed_batched": ([u32, vp, u32, vp, u64, vp], i32),
+            "am_tune_set": ([u32, u32, u32, u32, u32], i32),
+            "am_tune_get": ([u32, u32, u32, u32], u32),
+            "am_tune_clear": ([], None),
+            "am_tune_load": ([ctypes.c_char_p], i32),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = res
         _LIB = L
+        # the autotuned launch shapes (amethyst_amd/autotune.py) unless
+        # AMETHYST_TUNE=0 asks for the built-in ones
+        if os.environ.get("AMETHYST_TUNE", "1") != "0" and os.path.exists(TUNE_TABLE):
+            if L.am_tune_load(TUNE_TABLE.encode()) < 0:
+                raise RuntimeError(f"malformed launch-shape table {TUNE_TABLE}")
     return _LIB
 
 
@@ -149,6 +162,25 @@ def outbits(op, bits):
 
 def pred_width(op, c, bits):
     return int(lib().am_pred_width(am_predicate(op, 0, c), bits))
+
+
+def tune_set(knob, bits, in_fmt, out_fmt, value):
+    _check(lib().am_tune_set(knob, bits, in_fmt, out_fmt, value), "am_tune_set")
+
+
+def tune_get(knob, bits, in_fmt, out_fmt=0):
+    return int(lib().am_tune_get(knob, bits, in_fmt, out_fmt))
+
+
+def tune_clear():
+    lib().am_tune_clear()
+
+
+def tune_load(path):
+    got = int(lib().am_tune_load(path.encode()))
+    if got < 0:
+        raise RuntimeError(f"cannot read launch-shape table {path}")
+    return got
 
 
 def workspace_bytes(max_elements):

@@ -17,6 +17,9 @@ int am_groupby_wide(const am::Col& ck, const am::Col& cv, const am_predicate* va
                     uint64_t* out_keys, uint64_t* out_sums, uint64_t* out_counts, uint64_t* result_dev, void* ws,
                     uint64_t ws_bytes, hipStream_t s);
 
+// autotuned launch shape for (knob, bits, in format, out format), or 0 (capi.hip)
+uint32_t am_tuned(uint32_t knob, uint32_t bits, uint32_t in_format, uint32_t out_format);
+
 namespace am {
 
 inline uint32_t tight_bytes(uint32_t bits)

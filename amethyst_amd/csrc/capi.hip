@@ -1,6 +1,9 @@
 // capi.hip -- helper entry points of the C ABI (sizes, width rules,
 // workspace set-up). Operators live with their kernels.
+#include <atomic>
 #include <cstddef>
+#include <cstdio>
+#include <cstring>
 
 #include "am_device.hpp"
 #include "am_host.hpp"
@@ -105,4 +108,59 @@ extern "C" int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream)
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_zero_words, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, (uint64_t*)ws, words);
     return check_launch();
+}
+
+// ------------------------------------------------------------ launch-shape table
+
+static std::atomic<uint16_t> g_tune[AM_KNOB_COUNT][65][5][5];
+
+uint32_t am_tuned(uint32_t knob, uint32_t bits, uint32_t in_format, uint32_t out_format)
+{
+    if (knob >= AM_KNOB_COUNT || bits > 64 || in_format > 4 || out_format > 4) return 0;
+    return g_tune[knob][bits][in_format][out_format].load(std::memory_order_relaxed);
+}
+
+extern "C" int am_tune_set(uint32_t knob, uint32_t bits, uint32_t in_format, uint32_t out_format, uint32_t value)
+{
+    if (knob >= AM_KNOB_COUNT || bits < 1 || bits > 64 || in_format > 4 || out_format > 4 || value > 65535)
+        return AM_E_INVALID;
+    g_tune[knob][bits][in_format][out_format].store((uint16_t)value, std::memory_order_relaxed);
+    return AM_OK;
+}
+
+extern "C" uint32_t am_tune_get(uint32_t knob, uint32_t bits, uint32_t in_format, uint32_t out_format)
+{
+    return am_tuned(knob, bits, in_format, out_format);
+}
+
+extern "C" void am_tune_clear(void)
+{
+    for (auto& k : g_tune)
+        for (auto& b : k)
+            for (auto& i : b)
+                for (auto& o : i) o.store(0, std::memory_order_relaxed);
+}
+
+extern "C" int am_tune_load(const char* path)
+{
+    if (!path) return -1;
+    FILE* f = fopen(path, "r");
+    if (!f) return -1;
+    char line[256];
+    int count = 0;
+    while (fgets(line, sizeof line, f)) {
+        char* h = strchr(line, '#');
+        if (h) *h = 0;
+        unsigned k, b, fi, fo, v;
+        char extra;
+        const int got = sscanf(line, "%u %u %u %u %u %c", &k, &b, &fi, &fo, &v, &extra);
+        if (got <= 0) continue; // blank or comment
+        if (got != 5 || am_tune_set(k, b, fi, fo, v) != AM_OK) {
+            fclose(f);
+            return -1;
+        }
+        count++;
+    }
+    fclose(f);
+    return count;
 }

@@ -826,6 +826,11 @@ static int compress_launch(const am_column* in, const uint64_t* mask, const am_p
             if (co.kind != K_STRIDED && a.R > rs) a.R = rs; // packed outputs stage a whole tile
             static_assert(CD_STAGE_C >= CD_WAVES * 64 * CD_RMAX * 8, "container tile words fit the stage");
             if (a.R > CD_RMAX) a.R = CD_RMAX;
+            // a tuned chunk count, within the stage limits above and keeping
+            // tiles of >= 1024 elements (the descriptor area is sized for that)
+            const uint32_t rmin = (4 + a.eps - 1) / a.eps;
+            if (const uint32_t tr = am_tuned(AM_KNOB_COMPRESS_CHUNKS, in->bits, in->format, out->format))
+                if (tr < a.R) a.R = tr < rmin ? rmin : tr;
             if (a.R < 1) a.R = 1;
             a.lds16 = ((uintptr_t)ci.p & 15) == 0;
             a.out_vec = ((uintptr_t)co.p & (4 * co.cbytes - 1)) == 0;

@@ -114,9 +114,9 @@ struct BinWords {
 // takes BA_PER words, loads them all before computing any (more bytes in
 // flight per wave than one word per thread). AND/OR/XOR at equal width work
 // on whole words under the fields' mask (dead bits stay zero).
-constexpr uint32_t BA_PER = 4;
+constexpr uint32_t BA_PER_DEFAULT = 4; // words per thread (2, 4 or 8: AM_KNOB_BINARY_WORDS)
 
-template <int OP>
+template <int OP, uint32_t BA_PER>
 __global__ __launch_bounds__(256) void k_binary_aligned(BinWords g)
 {
     constexpr bool BITWISE = OP == AM_OP_AND || OP == AM_OP_OR || OP == AM_OP_XOR;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_binary_aligned(BinWords g)
 // Unaligned columns: a thread builds BA_PER output words (one per 256-word
 // sub-block of its block), loading the input words of all of them before
 // combining any.
-template <int OP>
+template <int OP, uint32_t BA_PER>
 __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
 {
     const uint64_t n = g.out.n;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
 }
 
 template <int OP>
-static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s)
+static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s, uint32_t fmt_a, uint32_t fmt_o)
 {
     auto field = [](const Col& c) { return c.kind == K_SLAB ? c.bits : 8 * c.cbytes; };
     if (a.kind == K_BITSTREAM || o.kind == K_BITSTREAM) return false;
@@ -226,15 +226,24 @@ static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s
     g.a.eps_magic = g.eps_i > 1 ? (uint32_t)((1ull << 32) / g.eps_i + 1) : 0;
     g.words_i = (a.n + g.eps_i - 1) / g.eps_i;
     g.words_o = (o.n + g.eps_o - 1) / g.eps_o;
+    const uint32_t tp = am_tuned(AM_KNOB_BINARY_WORDS, a.bits, fmt_a, fmt_o);
+    const uint32_t per = tp >= 8 ? 8 : (tp >= 4 || tp == 0 ? BA_PER_DEFAULT : 2);
+    auto grid = [&](uint32_t pw) { return dim3((uint32_t)((g.words_o + 256 * pw - 1) / (256 * pw))); };
     if (g.eps_o == g.eps_i) {
         if (g.fi == g.fo && a.mask == o.mask) // equal field layout
             for (uint32_t j = 0; j < g.eps_i; j++) g.fmask |= a.mask << (j * g.fi);
-        const dim3 grid((uint32_t)((g.words_o + 256 * BA_PER - 1) / (256 * BA_PER)));
-        hipLaunchKernelGGL((k_binary_aligned<OP>), grid, dim3(256), 0, s, g);
+        switch (per) {
+            case 2: hipLaunchKernelGGL((k_binary_aligned<OP, 2>), grid(2), dim3(256), 0, s, g); break;
+            case 8: hipLaunchKernelGGL((k_binary_aligned<OP, 8>), grid(8), dim3(256), 0, s, g); break;
+            default: hipLaunchKernelGGL((k_binary_aligned<OP, 4>), grid(4), dim3(256), 0, s, g); break;
+        }
         return true;
     }
-    const dim3 grid((uint32_t)((g.words_o + 256 * BA_PER - 1) / (256 * BA_PER)));
-    hipLaunchKernelGGL((k_binary_words<OP>), grid, dim3(256), 0, s, g);
+    switch (per) {
+        case 2: hipLaunchKernelGGL((k_binary_words<OP, 2>), grid(2), dim3(256), 0, s, g); break;
+        case 8: hipLaunchKernelGGL((k_binary_words<OP, 8>), grid(8), dim3(256), 0, s, g); break;
+        default: hipLaunchKernelGGL((k_binary_words<OP, 4>), grid(4), dim3(256), 0, s, g); break;
+    }
     return true;
 }
 
@@ -355,10 +364,10 @@ static bool launch_stream(const Col& a, const Col& b, const Col& o, hipStream_t 
 }
 
 template <int OP>
-static void launch_ew(const Col& a, const Col& b, const Col& o, hipStream_t s)
+static void launch_ew(const Col& a, const Col& b, const Col& o, hipStream_t s, uint32_t fmt_a, uint32_t fmt_o)
 {
     if constexpr (OP != OP_COPY) {
-        if (launch_words<OP>(a, b, o, s)) return;
+        if (launch_words<OP>(a, b, o, s, fmt_a, fmt_o)) return;
         if (launch_stream<OP>(a, b, o, s)) return;
     }
     uint64_t units = o.kind == K_STRIDED ? o.n : (o.kind == K_SLAB ? (o.n + o.eps - 1) / o.eps : (o.n * o.bits + 63) / 64);
@@ -391,7 +400,7 @@ extern "C" int am_convert(const am_column* in, am_column* out, void* stream)
     if ((st = make_col(out, &o))) return st;
     if (a.n != o.n) return AM_E_INVALID;
     if (o.n == 0) return AM_OK;
-    launch_ew<OP_COPY>(a, a, o, (hipStream_t)stream);
+    launch_ew<OP_COPY>(a, a, o, (hipStream_t)stream, in->format, out->format);
     return check_launch();
 }
 
@@ -406,17 +415,17 @@ extern "C" int am_binary(uint32_t op, const am_column* a, const am_column* b, am
     if (o.n == 0) return AM_OK;
     hipStream_t s = (hipStream_t)stream;
     switch (op) {
-        case AM_OP_ADD: launch_ew<AM_OP_ADD>(ca, cb, o, s); break;
-        case AM_OP_MUL: launch_ew<AM_OP_MUL>(ca, cb, o, s); break;
-        case AM_OP_MOD: launch_ew<AM_OP_MOD>(ca, cb, o, s); break;
-        case AM_OP_DIV: launch_ew<AM_OP_DIV>(ca, cb, o, s); break;
-        case AM_OP_MATH: launch_ew<AM_OP_MATH>(ca, cb, o, s); break;
-        case AM_OP_BOOL_SIMPLE: launch_ew<AM_OP_BOOL_SIMPLE>(ca, cb, o, s); break;
-        case AM_OP_BOOL_COMPLEX: launch_ew<AM_OP_BOOL_COMPLEX>(ca, cb, o, s); break;
-        case AM_OP_AND: launch_ew<AM_OP_AND>(ca, cb, o, s); break;
-        case AM_OP_OR: launch_ew<AM_OP_OR>(ca, cb, o, s); break;
-        case AM_OP_XOR: launch_ew<AM_OP_XOR>(ca, cb, o, s); break;
-        case AM_OP_SUB: launch_ew<AM_OP_SUB>(ca, cb, o, s); break;
+        case AM_OP_ADD: launch_ew<AM_OP_ADD>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_MUL: launch_ew<AM_OP_MUL>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_MOD: launch_ew<AM_OP_MOD>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_DIV: launch_ew<AM_OP_DIV>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_MATH: launch_ew<AM_OP_MATH>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_BOOL_SIMPLE: launch_ew<AM_OP_BOOL_SIMPLE>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_BOOL_COMPLEX: launch_ew<AM_OP_BOOL_COMPLEX>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_AND: launch_ew<AM_OP_AND>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_OR: launch_ew<AM_OP_OR>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_XOR: launch_ew<AM_OP_XOR>(ca, cb, o, s, a->format, out->format); break;
+        case AM_OP_SUB: launch_ew<AM_OP_SUB>(ca, cb, o, s, a->format, out->format); break;
         default: return AM_E_INVALID;
     }
     return check_launch();

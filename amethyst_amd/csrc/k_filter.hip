@@ -502,6 +502,7 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
         uint64_t r = a.in_words / (256ull * 8192);
         if (r < 2) r = 2;
         a.R = (uint32_t)(r > FW_RMAX ? FW_RMAX : r);
+        if (const uint32_t tr = am_tuned(AM_KNOB_FILTER_WORDS, c.bits, in->format, 0)) a.R = tr > FW_RMAX ? FW_RMAX : tr;
         if (c.bits <= 32) a.tp = make_top_pred(pr, c.bits, a.eps);
         a.mask_out = mask_out;
         const uint64_t TW = 256ull * a.R;
@@ -545,9 +546,12 @@ extern "C" int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t
     if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;
     uint64_t units = c.kind == K_SLAB ? (c.n + c.eps - 1) / c.eps
                    : (c.kind == K_BITSTREAM && c.bits <= 32 ? (c.n * c.bits + 63) / 64 : (c.n + 3) / 4);
+    // at most 2048 blocks of partial sums, or the tuned count (64..8192)
+    uint32_t bcap = am_tuned(AM_KNOB_SUM_BLOCKS, c.bits, in->format, 0);
+    bcap = bcap ? (bcap < 64 ? 64 : (bcap > 8192 ? 8192 : bcap)) : 2048;
     uint64_t blocks = (units + 256 * SUM_UNROLL - 1) / (256 * SUM_UNROLL);
     if (blocks < 1) blocks = 1;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > bcap) blocks = bcap;
     uint64_t* partial = ws_small(ws);
     hipStream_t s = (hipStream_t)stream;
     const bool wide = c.bits > 32;
@@ -575,7 +579,7 @@ extern "C" int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t
         if (sw.K < SUM_UNROLL) sw.K = SUM_UNROLL;
         uint64_t sb = (sw.words + 256 * SUM_UNROLL - 1) / (256 * SUM_UNROLL);
         if (sb < 1) sb = 1;
-        if (sb > 2048) sb = 2048;
+        if (sb > bcap) sb = bcap;
         hipLaunchKernelGGL(k_sum_swar, dim3((uint32_t)sb), b, 0, s, sw, partial);
         hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partial, (uint32_t)sb, sum_dev);
         return check_launch();

@@ -365,6 +365,8 @@ extern "C" int am_groupby(const am_column* keys, const am_column* values, const 
     a.fold_tiles = (uint32_t)((1ull << r) / tile); // r >= 15 for vbits <= 32, so >= 8 steps
     uint64_t maxb = WS_GBPART_BYTES / (16ull * G);
     if (maxb > 2048) maxb = 2048;
+    if (const uint32_t tb = am_tuned(AM_KNOB_GROUPBY_BLOCKS, keys->bits, keys->format, values->format))
+        if (tb < maxb) maxb = tb < 64 ? 64 : tb;
     const uint64_t tiles = (ck.n + tile - 1) / tile;
     uint64_t blocks = tiles < maxb ? (tiles ? tiles : 1) : maxb;
     const uint64_t tiles_per_block = tiles ? (tiles + blocks - 1) / blocks : 1;

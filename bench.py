@@ -150,8 +150,14 @@ def sweep(am, torch, timer, reps):
     stats = am.device_words(2)
     mask_words = (n + 63) // 64
 
-    def rec(d, ms, byt):
+    def rec(d, ms, byt, fn=None, key=None):
         d.update({"n": n, "ms": ms, "GBs": gbytes(byt, ms), "times": list(timer.last)})
+        tv = am.tune_get(*key) if key else 0
+        if tv:  # a tuned launch shape: time the built-in one beside it
+            am.tune_set(*key, 0)
+            d["ms_default"] = round(timer.median_ms(fn, reps), 4)
+            am.tune_set(*key, tv)
+            d["tuned"] = {"knob": key[0], "value": tv}
         ops.append(d)
 
     for w in WIDTHS:
@@ -164,8 +170,10 @@ def sweep(am, torch, timer, reps):
 
         # filter -> mask
         mo = am.device_words(mask_words)
-        ms = timer.median_ms(lambda: am.filter(a, am.LT, 1 << (w - 1), mask_out=mo), reps)
-        rec({"op": "filter", "w": w, "in": "slab", "out": "mask"}, ms, in_b + mask_words * 8)
+        f = lambda: am.filter(a, am.LT, 1 << (w - 1), mask_out=mo)
+        ms = timer.median_ms(f, reps)
+        rec({"op": "filter", "w": w, "in": "slab", "out": "mask"}, ms, in_b + mask_words * 8, f,
+            (am.KNOB_FILTER_WORDS, w, am.SLAB, 0))
 
         # fused filter + compressstore at 1/50/99 % selectivity, three output formats
         for frac in (0.01, 0.5, 0.99):
@@ -176,10 +184,10 @@ def sweep(am, torch, timer, reps):
                 am.compressstore(a, ob, fo, pred=(am.LT, c), ws=ws, out=out, stats=stats)
                 torch.cuda.synchronize()
                 k = int(am.u64(stats)[0])
-                ms = timer.median_ms(lambda: am.compressstore(a, ob, fo, pred=(am.LT, c), ws=ws, out=out, stats=stats),
-                                     reps)
+                f = lambda: am.compressstore(a, ob, fo, pred=(am.LT, c), ws=ws, out=out, stats=stats)
+                ms = timer.median_ms(f, reps)
                 rec({"op": "filter+compressstore", "w": w, "sel": frac, "in": "slab", "out": label, "ob": ob},
-                    ms, in_b + am.packed_words(fo, ob, k) * 8)
+                    ms, in_b + am.packed_words(fo, ob, k) * 8, f, (am.KNOB_COMPRESS_CHUNKS, w, am.SLAB, fo))
                 del out
 
         # mask-driven compressstore (reference semantics: slab -> slab same w):
@@ -200,26 +208,30 @@ def sweep(am, torch, timer, reps):
                 am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats)
                 torch.cuda.synchronize()
                 k = int(am.u64(stats)[0])
-                ms = timer.median_ms(lambda: am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats), reps)
+                f = lambda: am.compressstore(a, w, am.SLAB, mask=m, ws=ws, out=out, stats=stats)
+                ms = timer.median_ms(f, reps)
                 out_b = am.packed_words(am.SLAB, w, k) * 8
                 need = needed_line_bytes(torch, m, n, eps)
                 rec({"op": "compressstore(mask)", "w": w, "sel": frac, "mask": shape, "in": "slab",
                      "out": "slab_same_w", "needed_GBs": round(gbytes(need + mask_words * 8 + out_b, ms), 1)},
-                    ms, in_b + mask_words * 8 + out_b)
+                    ms, in_b + mask_words * 8 + out_b, f, (am.KNOB_COMPRESS_CHUNKS, w, am.SLAB, am.SLAB))
                 del out, m
 
         # binary add (w+1) and and (w)
         for op, name in ((am.ADD, "add"), (am.AND, "and")):
             ob = am.outbits(op, w)
             out = am.Column(n, ob, am.SLAB)
-            ms = timer.median_ms(lambda: am.binary(op, a, b, out=out), reps)
-            rec({"op": name, "w": w, "in": "slab", "out": f"slab w={ob}"}, ms, 2 * in_b + out.words * 8)
+            f = lambda: am.binary(op, a, b, out=out)
+            ms = timer.median_ms(f, reps)
+            rec({"op": name, "w": w, "in": "slab", "out": f"slab w={ob}"}, ms, 2 * in_b + out.words * 8, f,
+                (am.KNOB_BINARY_WORDS, w, am.SLAB, am.SLAB))
             del out
 
         # u64 sum
         s = am.device_words(1)
-        ms = timer.median_ms(lambda: am.sum(a, ws=ws, out=s), reps)
-        rec({"op": "sum", "w": w, "in": "slab"}, ms, in_b + 8)
+        f = lambda: am.sum(a, ws=ws, out=s)
+        ms = timer.median_ms(f, reps)
+        rec({"op": "sum", "w": w, "in": "slab"}, ms, in_b + 8, f, (am.KNOB_SUM_BLOCKS, w, am.SLAB, 0))
 
         # bitstream variants at w = 11
         if w == 11:
@@ -231,8 +243,9 @@ def sweep(am, torch, timer, reps):
             out = am.Column(n, w + 1, am.BITSTREAM)
             ms = timer.median_ms(lambda: am.binary(am.ADD, ab, bb, out=out), reps)
             rec({"op": "add", "w": w, "in": "bitstream", "out": "bitstream w=12"}, ms, 2 * ib + out.words * 8)
-            ms = timer.median_ms(lambda: am.sum(ab, ws=ws, out=s), reps)
-            rec({"op": "sum", "w": w, "in": "bitstream"}, ms, ib + 8)
+            f = lambda: am.sum(ab, ws=ws, out=s)
+            ms = timer.median_ms(f, reps)
+            rec({"op": "sum", "w": w, "in": "bitstream"}, ms, ib + 8, f, (am.KNOB_SUM_BLOCKS, w, am.BITSTREAM, 0))
             del ab, bb, out
         del a, b
         torch.cuda.empty_cache()

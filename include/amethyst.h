@@ -211,6 +211,40 @@ int am_gen_mask_clusters_ref(float selectivity, uint64_t cluster_count, uint64_t
  * CUDA_QUICKTIME (src/cuda_time.cuh:43-58). */
 int am_stream_copy(const uint64_t* src, uint64_t* dst, uint64_t nwords, void* stream);
 
+/* ---------------------------------------------------------------- launch-shape table */
+
+/* Launch shapes the autotuner chooses (amethyst_amd/autotune.py sweeps them
+ * on the GPU and writes amethyst_amd/tune_table.txt). An entry overrides the
+ * built-in shape for one (knob, bits, in_format, out_format); bits is the
+ * input width (the key width for group-by) and out_format is 0 where the
+ * operator has no output column (filter, sum). Values outside a knob's
+ * range are clamped to it. The table is process-wide host memory, read at
+ * every launch; set it before launching work that should use it. Counterpart
+ * of the reference's per-width grid x block sweep, which picks the best
+ * launch shape per width at benchmark time (src/common_code.cuh:179-225,
+ * src/utils.cuh:209-257). */
+typedef enum {
+    AM_KNOB_FILTER_WORDS = 0,    /* filter: 64-word chunks per wave per tile, 1..8 */
+    AM_KNOB_COMPRESS_CHUNKS = 1, /* compaction (slab/container input): 64-word chunks per wave per tile, 1..8 */
+    AM_KNOB_BINARY_WORDS = 2,    /* binary ops on slab/container words: output words per thread, 2, 4 or 8 */
+    AM_KNOB_SUM_BLOCKS = 3,      /* sum: blocks of the partial-sum launch, 64..8192 */
+    AM_KNOB_GROUPBY_BLOCKS = 4,  /* group-by over <= 12-bit keys: row blocks, 64..2048 */
+    AM_KNOB_PIPELINE_TILE = 5,   /* batched chain: elements per lane per tile (8 or 16) for variant 0 */
+    AM_KNOB_COUNT = 6
+} am_knob;
+
+/* value 0 removes the entry. AM_E_INVALID for an unknown knob, width or format. */
+int am_tune_set(uint32_t knob, uint32_t bits, uint32_t in_format, uint32_t out_format, uint32_t value);
+/* The entry, or 0 when there is none (the built-in shape is used). */
+uint32_t am_tune_get(uint32_t knob, uint32_t bits, uint32_t in_format, uint32_t out_format);
+/* Remove every entry. */
+void am_tune_clear(void);
+/* Read entries from a text file: one "knob bits in_format out_format value"
+ * per line, '#' starts a comment. Returns the number of entries read, or -1
+ * if the file cannot be opened or a line is malformed (entries before it
+ * stay set). */
+int am_tune_load(const char* path);
+
 /* ---------------------------------------------------------------- operators */
 
 /* Convert a column between formats/widths: out[i] = in[i] & mask(out.bits).

@@ -105,3 +105,33 @@ def test_status_strings_and_errors(L):
     assert L.am_groupby_workspace_bytes(1, 25, 20) == 0 and L.am_groupby_workspace_bytes(1, 10, 33) == 0
     # missing workspace
     assert L.am_sum(ctypes.byref(vals), dummy, None, 0, None) == 4
+
+
+def test_tune_table_parsing(L, tmp_path):
+    """The launch-shape table's text format and bounds (host only)."""
+    import amethyst_amd as am
+    saved = [(k, b, fi, fo, L.am_tune_get(k, b, fi, fo)) for k in range(6) for b in range(1, 65)
+             for fi in range(5) for fo in range(5)]
+    saved = [x for x in saved if x[4]]
+    try:
+        L.am_tune_clear()
+        assert L.am_tune_get(am.KNOB_FILTER_WORDS, 11, am.SLAB, 0) == 0
+        good = tmp_path / "t.txt"
+        good.write_text("# comment\n\n0 11 1 0 4   # filter\n1 11 1 4 6\n3 17 2 0 1024\n")
+        assert L.am_tune_load(str(good).encode()) == 3
+        assert L.am_tune_get(0, 11, 1, 0) == 4 and L.am_tune_get(1, 11, 1, 4) == 6 and L.am_tune_get(3, 17, 2, 0) == 1024
+        assert L.am_tune_set(0, 11, 1, 0, 0) == 0 and L.am_tune_get(0, 11, 1, 0) == 0
+        assert L.am_tune_set(6, 11, 1, 0, 1) == 1      # unknown knob
+        assert L.am_tune_set(0, 65, 1, 0, 1) == 1      # width
+        assert L.am_tune_set(0, 11, 5, 0, 1) == 1      # format
+        bad = tmp_path / "b.txt"
+        bad.write_text("0 11 1 0 4\n0 11 1\n")
+        assert L.am_tune_load(str(bad).encode()) == -1
+        assert L.am_tune_load(str(tmp_path / "missing.txt").encode()) == -1
+        # the committed table, if present, parses
+        if os.path.exists(am.TUNE_TABLE):
+            assert L.am_tune_load(am.TUNE_TABLE.encode()) >= 0
+    finally:
+        L.am_tune_clear()
+        for k, b, fi, fo, v in saved:
+            L.am_tune_set(k, b, fi, fo, v)
