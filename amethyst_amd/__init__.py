@@ -62,7 +62,7 @@ class am_predicate(ctypes.Structure):
 
 class am_pipeline_col(ctypes.Structure):
     _fields_ = [("a", am_column), ("b", am_column), ("s", am_column), ("pred", am_predicate),
-                ("stats", ctypes.c_void_p)]
+                ("stats", ctypes.c_void_p), ("variant", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 def build(verbose=False):
@@ -112,8 +112,8 @@ def lib():
             "am_groupby": ([colp, colp, predp, u32, vp, vp, vp, vp, vp, u64, vp], i32),
             "am_groupby_workspace_bytes": ([u64, u32, u32], u64),
             "am_pipeline_workspace_bytes": ([vp, u32], u64),
-            "am_pipeline_prepare": ([vp, u32, u32, vp, u64, vp], i32),
-            "am_pipeline_run": ([u32, u32, u32, u32, vp, u64, vp], i32),
+            "am_pipeline_prepare": ([vp, u32, vp, u64, vp, vp], i32),
+            "am_pipeline_run": ([vp, u64, u32, vp], i32),
             "am_plan_scratch_bytes": ([u32], u64),
             "am_plan_features": ([vp, u32, vp, vp, vp], i32),
             "am_plan_infer": ([vp, u32, vp, u32, vp, vp, ctypes.c_float, vp, vp, vp, vp], i32),
@@ -426,16 +426,18 @@ class PipelineBatch:
     """A prepared batch of fused filter -> compressstore(a+b) -> sum chains.
 
     columns: list of (a Column, b Column, (pred_op, c), s_fmt, s_bits).
-    run() launches one persistent kernel over every tile of every column
-    (capturable in a CUDA/HIP graph); results: per column the output Column
-    s (capacity n) and stats {count, overflow, total, 0}.
+    variant: elements per lane per tile for every column (8 or 16, 0 = the
+    launch-shape table's choice), or a list with one per column.
+    run() is one launch over every tile of every column (capturable in a
+    HIP graph); results: per column the output Column s (capacity n) and
+    stats {count, overflow, total, 0}.
     """
 
-    def __init__(self, columns, variant=16, outs=None):
+    def __init__(self, columns, variant=16, outs=None, stream=None):
         torch = _torch()
         n = len(columns)
         self.cols = columns
-        self.variant = variant
+        variants = list(variant) if isinstance(variant, (list, tuple, np.ndarray)) else [variant] * n
         self.outs = []
         self.stats = device_words(4 * n)
         arr = (am_pipeline_col * n)()
@@ -444,23 +446,35 @@ class PipelineBatch:
             out = outs[i] if outs is not None else Column(a.n, sbits, sfmt)
             self.outs.append(out)
             arr[i] = am_pipeline_col(a.c(), b.c(), out.c(), am_predicate(pred[0], 0, pred[1]),
-                                     self.stats.data_ptr() + 32 * i)
+                                     self.stats.data_ptr() + 32 * i, int(variants[i]), 0)
             ns[i] = a.n
         self.arr = arr
-        self.in_format = columns[0][0].fmt
         self.ws_bytes = int(lib().am_pipeline_workspace_bytes(ns, n))
         self.ws = device_words((self.ws_bytes + 7) // 8)
-        torch.cuda.synchronize()
         tt = ctypes.c_uint32(0)
-        _check(lib().am_pipeline_prepare(arr, n, variant, ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes,
-                                         ctypes.byref(tt)), "am_pipeline_prepare")
+        _check(lib().am_pipeline_prepare(arr, n, ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes,
+                                         ctypes.byref(tt), _stream(stream)), "am_pipeline_prepare")
         self.total_tiles = tt.value
         self.n = n
 
     def run(self, stream=None):
-        _check(lib().am_pipeline_run(self.n, self.total_tiles, self.in_format, self.variant,
-                                     ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes, _stream(stream)),
-               "am_pipeline_run")
+        _check(lib().am_pipeline_run(ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes, self.total_tiles,
+                                     _stream(stream)), "am_pipeline_run")
+
+    def capture(self):
+        """A HIP graph holding one run (torch.cuda.CUDAGraph; replay() to run).
+        The batch is run once eagerly on a side stream first."""
+        torch = _torch()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.run()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        return g
 
     def results(self):
         st = u64(self.stats).reshape(self.n, 4)

@@ -5,7 +5,8 @@ Plan space, per column (none of it changes any integer arithmetic):
   * output width: w+1 (the add rule, always safe), or the sampled maximum
     width "est" / "est+1" (narrower, fewer bytes; a value that does not fit
     is detected on the device and the column re-runs at w+1);
-  * kernel variant: 2048- or 4096-element tiles (J = 8 or 16).
+  * kernel variant: 2048- or 4096-element tiles (J = 8 or 16), per column
+    within the batch's one launch.
 
 The planner is an MLP (32 -> 64 -> 64 -> 12) over per-column statistics that
 the HIP kernel k_plan_stats computes; it predicts each plan's log-time
@@ -114,49 +115,38 @@ def est_bits(feat_row):
 # ----------------------------------------------------------------- execution
 
 def run_batch(cols, plans, feats, reps=0):
-    """Run columns with per-column plans (grouped by kernel variant, overflowed
-    columns re-run at w+1). Returns (outs, stats, seconds of the timed reps)."""
+    """Run columns with per-column plans in one launch (each column's tile
+    size is part of its plan); columns whose values did not fit the planned
+    width re-run at w+1. Returns (outs, stats, seconds per timed rep, reruns)."""
     import amethyst_amd as am
     torch = am._torch()
-    groups = {}
+    spec, variants = [], []
     for i, p in enumerate(plans):
-        groups.setdefault(PLANS[p][2], []).append(i)
-    batches = []
-    for var, idx in groups.items():
-        spec = []
-        for i in idx:
-            a, b, pred = cols[i]
-            fmt, wr, _ = PLANS[plans[i]]
-            spec.append((a, b, pred, fmt, plan_width(wr, a.bits, est_bits(feats[i]))))
-        batches.append((idx, PipelineBatch(spec, variant=var)))
-    def go():
-        for _, bt in batches:
-            bt.run()
-    go()
+        a, b, pred = cols[i]
+        fmt, wr, var = PLANS[p]
+        spec.append((a, b, pred, fmt, plan_width(wr, a.bits, est_bits(feats[i]))))
+        variants.append(var)
+    bt = PipelineBatch(spec, variant=variants)
+    bt.run()
     torch.cuda.synchronize()
     secs = None
     if reps:
         t0 = time.perf_counter()
         for _ in range(reps):
-            go()
+            bt.run()
         torch.cuda.synchronize()
         secs = (time.perf_counter() - t0) / reps
-    outs, stats = [None] * len(cols), [None] * len(cols)
-    redo = []
-    for idx, bt in batches:
-        o, st = bt.results()
-        for j, i in enumerate(idx):
-            outs[i], stats[i] = o[j], st[j]
-            if int(st[j][1]):
-                redo.append(i)
+    o, st = bt.results()
+    outs, stats = list(o), [st[i] for i in range(len(cols))]
+    redo = [i for i in range(len(cols)) if int(stats[i][1])]
     if redo:  # verified fallback: values wider than the chosen width
         spec = [(cols[i][0], cols[i][1], cols[i][2], PLANS[plans[i]][0], cols[i][0].bits + 1) for i in redo]
-        bt = PipelineBatch(spec, variant=16)
-        bt.run()
+        bt2 = PipelineBatch(spec, variant=16)
+        bt2.run()
         torch.cuda.synchronize()
-        o, st = bt.results()
+        o2, st2 = bt2.results()
         for j, i in enumerate(redo):
-            outs[i], stats[i] = o[j], st[j]
+            outs[i], stats[i] = o2[j], st2[j]
     return outs, stats, secs, len(redo)
 
 

@@ -402,8 +402,9 @@ def batched_leg(am, torch, reps):
     uniform / Zipf(1.2) / sorted+noise; per column filter -> compressstore ->
     add second column -> sum. Plain: one operator launch at a time per column
     (count read back between stages). Batched: planner (HIP features + MLP, or
-    the analytic plan without trained weights) then one persistent launch per
-    kernel variant, replayed from a captured graph."""
+    the analytic plan without trained weights) then one launch for the whole
+    batch (each column's tile size per its plan), also replayed from a
+    captured graph."""
     from amethyst_amd import planner
     cols = []
     for i in range(512):
@@ -435,32 +436,18 @@ def batched_leg(am, torch, reps):
     byt = 0
     for (a, b, _), o, st in zip(cols, outs, stats):
         byt += a.words * 8 + b.words * 8 + am.packed_words(o.fmt, o.bits, int(st[0])) * 8
-    # graph replay of the batched launches (same plans, no fallback needed now)
-    groups = {}
+    # graph replay of the batched launch (same plans; overflowed columns at w+1)
+    spec, variants = [], []
     for i, p in enumerate(plans):
-        groups.setdefault(planner.PLANS[p][2], []).append(i)
-    batches = []
-    for var, idx in groups.items():
-        spec = []
-        for i in idx:
-            a, b, pred = cols[i]
-            fmt, wr, _ = planner.PLANS[plans[i]]
-            w = planner.plan_width(wr, a.bits, planner.est_bits(fr[i]))
-            if int(stats[i][1]):
-                w = a.bits + 1
-            spec.append((a, b, pred, fmt, w))
-        batches.append(am.PipelineBatch(spec, variant=var))
-    g = torch.cuda.CUDAGraph()
-    s = torch.cuda.Stream()
-    s.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        for bt in batches:
-            bt.run()
-    torch.cuda.current_stream().wait_stream(s)
-    torch.cuda.synchronize()
-    with torch.cuda.graph(g):
-        for bt in batches:
-            bt.run()
+        a, b, pred = cols[i]
+        fmt, wr, var = planner.PLANS[p]
+        w = planner.plan_width(wr, a.bits, planner.est_bits(fr[i]))
+        if int(stats[i][1]):
+            w = a.bits + 1
+        spec.append((a, b, pred, fmt, w))
+        variants.append(var)
+    bt = am.PipelineBatch(spec, variant=variants)
+    g = bt.capture()
     g.replay()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -469,8 +456,7 @@ def batched_leg(am, torch, reps):
         g.replay()
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t0) / nrep
-    res_ok = all(int(bt.results()[1][j][2]) == plain_totals[i] for bt, (var, idx) in zip(batches, groups.items())
-                 for j, i in enumerate(idx))
+    res_ok = [int(x[2]) for x in bt.results()[1]] == plain_totals
     return {"columns": len(cols), "rows": rows, "plain_s": round(t_plain, 5), "batched_s": round(t_batch, 5),
             "graph_s": round(t_graph, 5), "plan_s": round(t_plan, 5), "planner": kind,
             "bytes": byt, "GBs": round(byt / t_batch / 1e9, 1), "pct_peak": round(100 * byt / t_batch / 1e9 / PEAK_GBS, 1),

@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define AM_ABI_VERSION 1
+#define AM_ABI_VERSION 2
 
 typedef enum {
     AM_OK = 0,
@@ -338,33 +338,41 @@ int am_groupby(const am_column* keys, const am_column* values, const am_predicat
 /* ---------------------------------------------------------------- batched operator chain */
 
 /* One column of a batch: mask = filter(a, pred); s = compressstore(a, mask)
- * + compressstore(b, mask) (add, width bits+1); total = sum(s). `s` gives the
- * output format (slab or bitstream), width and capacity (s.n); stats_dev
- * (device, 4 x uint64) receives {selected count, values that did not fit
- * s.bits, total, 0}. a and b: same n, bits (<= 31) and format. */
+ * + compressstore(b, mask) (add, width bits+1); total = the exact sum of the
+ * selected a+b (whatever s.bits is). `s` gives the output format (slab or
+ * bitstream), width and capacity (s.n); values wider than s.bits are stored
+ * masked and counted. stats (device, 4 x uint64) receives {selected count,
+ * values that did not fit s.bits, total, 0}. a and b: same n, bits (<= 31)
+ * and format (any format). variant: elements per lane per tile, 8 or 16
+ * (2048- or 4096-element tiles), or 0 for the launch-shape table's choice
+ * (AM_KNOB_PIPELINE_TILE), else 16. */
 typedef struct {
     am_column a;
     am_column b;
     am_column s;
     am_predicate pred;
     uint64_t* stats;
+    uint32_t variant;
+    uint32_t reserved;
 } am_pipeline_col;
 
 /* Workspace for a batch whose columns have lengths n[0..count). */
 uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t count);
 
-/* Validate the batch and upload its job table into the workspace
- * (synchronous); *total_tiles receives the tile count to pass to run.
- * variant = elements per lane per tile (8 or 16: 2048- or 4096-element
- * tiles). The reference chains operators only by format, one launch each
+/* Validate the batch and upload its job table into the workspace; waits for
+ * `stream` first (a previous run on it may still read the table), then
+ * copies synchronously. *total_tiles receives the grid size to pass to run.
+ * The reference chains operators only by format, one launch each
  * (src/op_chaining_demo.cu:137-152); here the chain is fused. */
-int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, uint32_t variant, void* ws, uint64_t ws_bytes,
-                        uint32_t* total_tiles);
+int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, void* ws, uint64_t ws_bytes,
+                        uint32_t* total_tiles, void* stream);
 
-/* Run a prepared batch: one persistent launch over every tile of every
- * column plus a one-block epoch update; no host work, graph-capturable. */
-int am_pipeline_run(uint32_t count, uint32_t total_tiles, uint32_t in_format, uint32_t variant, void* ws,
-                    uint64_t ws_bytes, void* stream);
+/* Run a prepared batch: ONE launch, one block per tile of every column
+ * (tiles numbered column by column), no host work, graph-capturable. The
+ * kernel checks the workspace's recorded batch against ws_bytes and
+ * total_tiles; on a mismatch it only writes stats[0] = UINT64_MAX for every
+ * column. A workspace that was zero-filled again must be prepared again. */
+int am_pipeline_run(void* ws, uint64_t ws_bytes, uint32_t total_tiles, void* stream);
 
 /* ---------------------------------------------------------------- learned planner */
 

@@ -930,3 +930,55 @@ def test_groupby_wide_keys(am, oracle_mod):
                     say("groupby-wide", kbits, vbits, n, dist, kf, vf, mode, pred)
                     _check_groupby(am, oracle_mod, keys, vals, kbits, vbits, kf, vf, mode, pred, ws,
                                    (kbits, vbits, n, kf, vf, mode, pred))
+
+
+def test_pipeline_mixed_tiles_one_launch(am, oracle_mod):
+    """Columns of both tile sizes and several input formats' worth of widths
+    in ONE launch, run repeatedly on one workspace; then a second batch
+    prepared on the same workspace."""
+    import torch
+    rng = np.random.default_rng(91)
+    cols, refs = _pipeline_cols(am, oracle_mod, rng, 18, 2)
+    variants = [(8, 16, 0)[i % 3] for i in range(len(cols))]
+    batch = am.PipelineBatch(cols, variant=variants)
+    for _ in range(3):
+        batch.run()
+    torch.cuda.synchronize()
+    outs, st = batch.results()
+    _check_pipeline(oracle_mod, refs, outs, st, "mixed")
+    cols2, refs2 = _pipeline_cols(am, oracle_mod, rng, 9, 1)
+    b2 = am.PipelineBatch(cols2, variant=8)
+    b2.ws, b2.ws_bytes = batch.ws, batch.ws_bytes  # reuse the first batch's workspace
+    import ctypes
+    tt = ctypes.c_uint32(0)
+    assert am.lib().am_pipeline_prepare(b2.arr, b2.n, ctypes.c_void_p(b2.ws.data_ptr()), b2.ws_bytes,
+                                        ctypes.byref(tt), am._stream()) == 0
+    b2.total_tiles = tt.value
+    b2.run()
+    b2.run()
+    torch.cuda.synchronize()
+    outs2, st2 = b2.results()
+    _check_pipeline(oracle_mod, refs2, outs2, st2, "second batch")
+
+
+def test_pipeline_run_rejects_mismatched_launch(am, oracle_mod):
+    """A run whose tile count or workspace size differs from the prepared
+    batch writes stats[0] = UINT64_MAX for every column and nothing else
+    (the kernel checks the recorded batch; every block exits normally)."""
+    import torch
+    rng = np.random.default_rng(92)
+    cols, refs = _pipeline_cols(am, oracle_mod, rng, 5, 1)
+    batch = am.PipelineBatch(cols, variant=16)
+    for tiles, wsb in ((batch.total_tiles + 1, batch.ws_bytes), (max(1, batch.total_tiles - 1), batch.ws_bytes),
+                       (batch.total_tiles, 256 + 65536 + 64)):  # less than the batch's job table and descriptors
+        batch.stats.zero_()
+        st = am.lib().am_pipeline_run(am.ctypes.c_void_p(batch.ws.data_ptr()), wsb, tiles, am._stream())
+        assert st == 0
+        torch.cuda.synchronize()
+        s = batch.results()[1]
+        assert all(int(x[0]) == 2**64 - 1 for x in s), (tiles, wsb)
+    batch.stats.zero_()
+    batch.run()  # the matching launch still works afterwards
+    torch.cuda.synchronize()
+    outs, st = batch.results()
+    _check_pipeline(oracle_mod, refs, outs, st, "after mismatches")
