@@ -42,9 +42,14 @@ def parse():
     p.add_argument("--reps", type=int, default=10, help="launches per sweep point")
     p.add_argument("--csv", default=None, metavar="DIR",
                    help="also write every timed launch of the sweep in the reference's CSV schema under DIR")
+    p.add_argument("--case", default=None, choices=sorted(CASES),
+                   help="run one workload --steps times and exit (for rocprofv3; prints its algorithmic bytes)")
     p.add_argument("--profile-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (reference point for HBM counters)")
     return p.parse_args()
+
+
+CASES = ("headline", "compress_slab_w4", "compress_slab_w11", "pipeline", "copy_1gib")
 
 
 def gbytes(b, ms):
@@ -465,6 +470,59 @@ def batched_leg(am, torch, reps):
             "fallback_reruns": redo, "totals_match_plain": bool(ok and res_ok)}
 
 
+def run_case(am, torch, name, steps):
+    """One workload, `steps` launches, for profiling: the headline compaction,
+    fused filter + compressstore into same-width slabs at 50 % selectivity
+    (w = 4, 11), the 512-column batched chain (one launch), or a plain 1 GiB
+    device copy (the reference point of the HBM byte counters)."""
+    n = N_FULL
+    if name == "copy_1gib":
+        src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        fn, byt = (lambda: dst.copy_(src)), 2 << 30
+    elif name == "pipeline":
+        from amethyst_amd import planner
+        cols = [planner.make_inputs(50_000 + i, planner.column_spec(50_000 + i)) for i in range(512)]
+        feat = planner.features(cols)
+        plans, _ = planner.Planner().choose(feat)
+        fr = feat.cpu().numpy()
+        spec, var = [], []
+        for i, p in enumerate(plans):
+            fmt, wr, v = planner.PLANS[p]
+            spec.append((cols[i][0], cols[i][1], cols[i][2], fmt, planner.plan_width(wr, cols[i][0].bits,
+                                                                                      planner.est_bits(fr[i]))))
+            var.append(v)
+        bt = am.PipelineBatch(spec, variant=var)
+        bt.run()
+        torch.cuda.synchronize()
+        st = bt.results()[1]
+        byt = sum(a.words * 8 + b.words * 8 + am.packed_words(o.fmt, o.bits, int(x[0])) * 8
+                  for (a, b, _), o, x in zip(cols, bt.outs, st))
+        fn = bt.run
+    else:
+        w = {"headline": 11, "compress_slab_w4": 4, "compress_slab_w11": 11}[name]
+        col = am.gen_uniform(42, n, w, am.SLAB)
+        ws = am.Workspace(n)
+        stats = am.device_words(2)
+        if name == "headline":
+            out = am.Column(n, 32, am.U32)
+            mo = am.device_words((n + 63) // 64)
+            fn = lambda: am.compressstore(col, 32, am.U32, pred=(am.LT, 1024), ws=ws, out=out, stats=stats, mask_out=mo)
+        else:
+            out = am.Column(n, w, am.SLAB)
+            fn = lambda: am.compressstore(col, w, am.SLAB, pred=(am.LT, 1 << (w - 1)), ws=ws, out=out, stats=stats)
+        fn()
+        torch.cuda.synchronize()
+        k = int(am.u64(stats)[0])
+        byt = col.words * 8 + out.words * 0 + am.packed_words(out.fmt, out.bits, k) * 8
+        if name == "headline":
+            byt += (n + 63) // 64 * 8
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    print(json.dumps({"case": name, "launches": steps, "algorithmic_bytes_per_launch": byt}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -476,6 +534,9 @@ def main():
     am.lib()
     torch.cuda.init()
     timer = Timer(torch)
+    if args.case:
+        run_case(am, torch, args.case, args.steps)
+        return
     if args.profile_copy:
         src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
