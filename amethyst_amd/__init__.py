@@ -94,6 +94,7 @@ def lib():
             "am_workspace_bytes": ([u64], u64),
             "am_workspace_init": ([vp, u64, vp], i32),
             "am_workspace_status": ([vp], i32),
+            "am_read_device": ([vp, vp, u64, vp], i32),
             "am_prng_fill_u32": ([u64, u64, vp, u64, vp], i32),
             "am_gen_uniform": ([u64, colp, vp], i32),
             "am_gen_from_cdf": ([u64, vp, u32, colp, vp], i32),
@@ -121,6 +122,9 @@ def lib():
             "am_matmul_workspace_bytes": ([vp, vp, vp, u32], u64),
             "am_matmul_packed": ([u32, vp, vp, u32, u32, u32, u32, vp, vp, vp, u64, vp], i32),
             "am_matmul_packed_batched": ([u32, vp, u32, vp, u64, vp], i32),
+            "am_matmul_plan_create": ([u32, vp, u32, vp, u64, vp, vp], i32),
+            "am_matmul_plan_run": ([vp, vp], i32),
+            "am_matmul_plan_destroy": ([vp], None),
             "am_tune_set": ([u32, u32, u32, u32, u32], i32),
             "am_tune_get": ([u32, u32, u32, u32], u32),
             "am_tune_clear": ([], None),
@@ -527,6 +531,41 @@ def matmul_packed_batched(bits, problems, want_i32=False, out_bits=None, stream=
            "am_matmul_packed_batched")
     outs.append(ws)  # keep the workspace alive until the caller synchronises
     return outs
+
+
+class MatmulPlan:
+    """A prepared batch of packed products (am_matmul_plan_*): the header is
+    uploaded once; run() launches the kernels only (graph-capturable).
+    problems as for matmul_packed_batched; outs like its return value."""
+
+    def __init__(self, bits, problems, want_i32=False, out_bits=None, stream=None):
+        torch = _torch()
+        n = len(problems)
+        Ms = (ctypes.c_uint32 * n)(*[p[2] for p in problems])
+        Ns = (ctypes.c_uint32 * n)(*[p[3] for p in problems])
+        Ks = (ctypes.c_uint32 * n)(*[p[4] for p in problems])
+        self.ws_bytes = int(lib().am_matmul_workspace_bytes(Ms, Ns, Ks, n))
+        self.ws = torch.empty((self.ws_bytes + 7) // 8, dtype=torch.int64, device="cuda")
+        self.outs, descs = [], (am_matmul_desc * n)()
+        for i, (A, B, M, N, K) in enumerate(problems):
+            ob = out_bits if out_bits else matmul_out_bits(bits, K)
+            cp = torch.empty(max(1, matrix_words(M, N, ob)), dtype=torch.int64, device="cuda")
+            ci = torch.empty(M * N, dtype=torch.int32, device="cuda") if want_i32 else None
+            descs[i] = am_matmul_desc(A.data_ptr(), B.data_ptr(), cp.data_ptr(), ci.data_ptr() if ci is not None else None,
+                                      M, N, K, ob)
+            self.outs.append((cp, ci, ob))
+        self.problems = problems  # keep the inputs alive
+        self.handle = ctypes.c_void_p()
+        _check(lib().am_matmul_plan_create(bits, descs, n, ctypes.c_void_p(self.ws.data_ptr()), self.ws_bytes,
+                                           ctypes.byref(self.handle), _stream(stream)), "am_matmul_plan_create")
+
+    def run(self, stream=None):
+        _check(lib().am_matmul_plan_run(self.handle, _stream(stream)), "am_matmul_plan_run")
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value and _LIB is not None:
+            _LIB.am_matmul_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
 
 
 def u64(t):

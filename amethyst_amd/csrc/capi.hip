@@ -100,6 +100,15 @@ extern "C" int am_workspace_status(const void* ws)
     return abort ? AM_E_ABORTED : AM_OK;
 }
 
+extern "C" int am_read_device(void* host_dst, const void* dev_src, uint64_t bytes, void* stream)
+{
+    if (bytes == 0) return AM_OK;
+    if (!host_dst || !dev_src) return AM_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return AM_E_HIP;
+    return hipStreamSynchronize(s) == hipSuccess ? AM_OK : AM_E_HIP;
+}
+
 extern "C" int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream)
 {
     if (!ws || ws_bytes < WS_FIXED) return AM_E_WORKSPACE;

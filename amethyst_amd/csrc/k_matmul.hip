@@ -23,8 +23,9 @@
 // one persistent launch per phase (work lists built on the host).
 #include <algorithm>
 #include <cstring>
-#include <mutex>
 #include <vector>
+
+#include <new>
 
 #include "am_device.hpp"
 #include "am_host.hpp"
@@ -473,45 +474,6 @@ extern "C" uint64_t am_matmul_workspace_bytes(const uint32_t* M, const uint32_t*
     return b;
 }
 
-// Pinned host staging for the per-call header (job list, packer offsets,
-// unpack block table): one asynchronous copy per call instead of pageable
-// copies, which would block the host until the stream is idle. Two slots per
-// device; a slot is rewritten only after the copy that last used it is done.
-struct MmUpload {
-    void* host[2] = {nullptr, nullptr};
-    size_t cap[2] = {0, 0};
-    hipEvent_t done[2] = {nullptr, nullptr};
-    uint32_t next = 0;
-};
-
-static std::mutex g_up_mu;
-static MmUpload g_up[64];
-
-// Copy `bytes` built by `fill(host_ptr)` to `dst` on stream `s` (AM_OK or AM_E_HIP).
-template <typename F>
-static int mm_upload(void* dst, size_t bytes, hipStream_t s, F fill)
-{
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return AM_E_HIP;
-    std::lock_guard<std::mutex> g(g_up_mu);
-    MmUpload& u = g_up[dev];
-    const uint32_t k = u.next;
-    u.next ^= 1;
-    if (!u.done[k] && hipEventCreateWithFlags(&u.done[k], hipEventDisableTiming) != hipSuccess) return AM_E_HIP;
-    if (u.host[k] && hipEventSynchronize(u.done[k]) != hipSuccess) return AM_E_HIP; // its last copy is done
-    if (u.cap[k] < bytes) {
-        if (u.host[k]) (void)hipHostFree(u.host[k]);
-        u.host[k] = nullptr;
-        u.cap[k] = 0;
-        const size_t want = std::max<size_t>(bytes, 1 << 16);
-        if (hipHostMalloc(&u.host[k], want, hipHostMallocDefault) != hipSuccess) return AM_E_HIP;
-        u.cap[k] = want;
-    }
-    fill((uint8_t*)u.host[k]);
-    if (hipMemcpyAsync(dst, u.host[k], bytes, hipMemcpyHostToDevice, s) != hipSuccess) return AM_E_HIP;
-    return hipEventRecord(u.done[k], s) == hipSuccess ? AM_OK : AM_E_HIP;
-}
-
 template <uint32_t BITS>
 static void mm_launch_unpack(const MatJob* djobs, const MatSeg* dsegs, uint32_t na, uint32_t nb, hipStream_t s)
 {
@@ -526,8 +488,24 @@ extern "C" uint32_t am_matmul_out_bits(uint32_t bits, uint32_t K)
     return width_of_host(vmax);
 }
 
-extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws,
-                                        uint64_t ws_bytes, void* stream)
+// Launch parameters of a prepared batch; the header (job list, packer word
+// offsets, unpack block table) lives in the workspace.
+struct am_matmul_plan {
+    uint32_t bits, count, na, nb;
+    uint64_t tiles, pwords;
+    uint32_t capb; // resident GEMM blocks (the persistent grid)
+    bool need_pack;
+    MatJob* djobs;
+    uint32_t* dword0;
+    MatSeg* dsegs;
+};
+
+// Validate, lay out the workspace and build the header on the host; the
+// header is copied to the device with a stream-ordered copy from `hdr`
+// (host memory the caller keeps until the copy has been issued: a copy from
+// pageable memory returns once the source has been read).
+static int mm_prepare(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws, uint64_t ws_bytes,
+                      am_matmul_plan& P, hipStream_t s)
 {
     if (bits < 1 || bits > 7) return AM_E_UNSUPPORTED_CONFIG;
     if (!descs || count == 0) return AM_E_INVALID;
@@ -592,40 +570,80 @@ extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* des
     }
     if (tiles >= (1ull << 32) || pwords >= (1ull << 32) || segs_a.size() + segs_b.size() >= (1ull << 31))
         return AM_E_UNSUPPORTED_CONFIG;
+    P.na = (uint32_t)segs_a.size();
+    P.nb = (uint32_t)segs_b.size();
     segs_a.insert(segs_a.end(), segs_b.begin(), segs_b.end());
-    hipStream_t s = (hipStream_t)stream;
-    MatJob* djobs = (MatJob*)((uint8_t*)ws + 4096);
-    uint32_t* dword0 = (uint32_t*)((uint8_t*)djobs + (uint64_t)count * sizeof(MatJob));
+    P.djobs = (MatJob*)((uint8_t*)ws + 4096);
+    P.dword0 = (uint32_t*)((uint8_t*)P.djobs + (uint64_t)count * sizeof(MatJob));
+    P.dsegs = (MatSeg*)(((uintptr_t)(P.dword0 + count) + 7) & ~(uintptr_t)7);
+    const size_t o_w0 = (uint8_t*)P.dword0 - (uint8_t*)P.djobs, o_seg = (uint8_t*)P.dsegs - (uint8_t*)P.djobs;
+    std::vector<uint8_t> hdr(o_seg + segs_a.size() * sizeof(MatSeg));
+    memcpy(hdr.data(), jobs.data(), count * sizeof(MatJob));
+    memcpy(hdr.data() + o_w0, word0.data(), count * 4);
+    memcpy(hdr.data() + o_seg, segs_a.data(), segs_a.size() * sizeof(MatSeg));
+    // (stream-ordered: earlier work on `s` that reads this workspace's
+    // header finishes first; `hdr` may be released once the call returns)
+    if (hipMemcpyAsync(P.djobs, hdr.data(), hdr.size(), hipMemcpyHostToDevice, s) != hipSuccess) return AM_E_HIP;
+    P.capb = resident_blocks(k_mm_gemm, MM_THREADS, 0);
+    P.bits = bits;
+    P.count = count;
+    P.tiles = tiles;
+    P.pwords = pwords;
+    P.need_pack = need_pack;
+    return AM_OK;
+}
 
-    MatSeg* dsegs = (MatSeg*)(((uintptr_t)(dword0 + count) + 7) & ~(uintptr_t)7);
-    const size_t o_w0 = (uint8_t*)dword0 - (uint8_t*)djobs, o_seg = (uint8_t*)dsegs - (uint8_t*)djobs;
-    const size_t hbytes = o_seg + segs_a.size() * sizeof(MatSeg);
-    const int up = mm_upload(djobs, hbytes, s, [&](uint8_t* h) {
-        memcpy(h, jobs.data(), count * sizeof(MatJob));
-        memcpy(h + o_w0, word0.data(), count * 4);
-        memcpy(h + o_seg, segs_a.data(), segs_a.size() * sizeof(MatSeg));
-    });
-    if (up != AM_OK) return up;
-    const uint32_t na = (uint32_t)(segs_a.size() - segs_b.size()), nb = (uint32_t)segs_b.size();
-    switch (bits) {
-        case 1: mm_launch_unpack<1>(djobs, dsegs, na, nb, s); break;
-        case 2: mm_launch_unpack<2>(djobs, dsegs, na, nb, s); break;
-        case 3: mm_launch_unpack<3>(djobs, dsegs, na, nb, s); break;
-        case 4: mm_launch_unpack<4>(djobs, dsegs, na, nb, s); break;
-        case 5: mm_launch_unpack<5>(djobs, dsegs, na, nb, s); break;
-        case 6: mm_launch_unpack<6>(djobs, dsegs, na, nb, s); break;
-        default: mm_launch_unpack<7>(djobs, dsegs, na, nb, s); break;
+static int mm_launch(const am_matmul_plan& P, hipStream_t s)
+{
+    switch (P.bits) {
+        case 1: mm_launch_unpack<1>(P.djobs, P.dsegs, P.na, P.nb, s); break;
+        case 2: mm_launch_unpack<2>(P.djobs, P.dsegs, P.na, P.nb, s); break;
+        case 3: mm_launch_unpack<3>(P.djobs, P.dsegs, P.na, P.nb, s); break;
+        case 4: mm_launch_unpack<4>(P.djobs, P.dsegs, P.na, P.nb, s); break;
+        case 5: mm_launch_unpack<5>(P.djobs, P.dsegs, P.na, P.nb, s); break;
+        case 6: mm_launch_unpack<6>(P.djobs, P.dsegs, P.na, P.nb, s); break;
+        default: mm_launch_unpack<7>(P.djobs, P.dsegs, P.na, P.nb, s); break;
     }
     auto grid_for = [](uint64_t items) { return (uint32_t)std::min<uint64_t>((items + 255) / 256, 65535); };
-    static uint32_t capb = 0;
-    if (!capb) capb = resident_blocks(k_mm_gemm, MM_THREADS, 0);
-    hipLaunchKernelGGL(k_mm_gemm, dim3((uint32_t)std::min<uint64_t>(tiles, capb)), dim3(MM_THREADS), 0, s, djobs, count,
-                       (uint32_t)tiles);
-    if (need_pack)
-        hipLaunchKernelGGL(k_mm_pack_rows, dim3(grid_for(pwords)), dim3(256), 0, s, djobs, count, dword0,
-                           (uint32_t)pwords);
+    hipLaunchKernelGGL(k_mm_gemm, dim3((uint32_t)std::min<uint64_t>(P.tiles, P.capb)), dim3(MM_THREADS), 0, s, P.djobs,
+                       P.count, (uint32_t)P.tiles);
+    if (P.need_pack)
+        hipLaunchKernelGGL(k_mm_pack_rows, dim3(grid_for(P.pwords)), dim3(256), 0, s, P.djobs, P.count, P.dword0,
+                           (uint32_t)P.pwords);
     return check_launch();
 }
+
+extern "C" int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws,
+                                        uint64_t ws_bytes, void* stream)
+{
+    am_matmul_plan P{};
+    const int st = mm_prepare(bits, descs, count, ws, ws_bytes, P, (hipStream_t)stream);
+    return st ? st : mm_launch(P, (hipStream_t)stream);
+}
+
+extern "C" int am_matmul_plan_create(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws,
+                                     uint64_t ws_bytes, am_matmul_plan** plan, void* stream)
+{
+    if (!plan) return AM_E_INVALID;
+    *plan = nullptr;
+    am_matmul_plan* P = new (std::nothrow) am_matmul_plan{};
+    if (!P) return AM_E_INVALID;
+    const int st = mm_prepare(bits, descs, count, ws, ws_bytes, *P, (hipStream_t)stream);
+    if (st) {
+        delete P;
+        return st;
+    }
+    *plan = P;
+    return AM_OK;
+}
+
+extern "C" int am_matmul_plan_run(const am_matmul_plan* plan, void* stream)
+{
+    if (!plan) return AM_E_INVALID;
+    return mm_launch(*plan, (hipStream_t)stream);
+}
+
+extern "C" void am_matmul_plan_destroy(am_matmul_plan* plan) { delete plan; }
 
 extern "C" int am_matmul_packed(uint32_t bits, const uint64_t* A, const uint64_t* B, uint32_t M, uint32_t N, uint32_t K,
                                 uint32_t out_bits, uint64_t* C_packed, int32_t* C_i32, void* ws, uint64_t ws_bytes,

@@ -148,6 +148,11 @@ uint64_t am_gen_bitmask(uint32_t bits);
 uint64_t am_workspace_bytes(uint64_t max_elements);
 int am_workspace_init(void* ws, uint64_t ws_bytes, void* stream);
 
+/* Copy `bytes` from device memory to host memory after the work queued on
+ * `stream` before it, and wait for the copy (e.g. an operator's statistics).
+ * Used by the C++ facade's count-returning wrappers. */
+int am_read_device(void* host_dst, const void* dev_src, uint64_t bytes, void* stream);
+
 /* Synchronous health check of a workspace: AM_OK, or AM_E_ABORTED when a
  * single-pass launch gave up waiting on a predecessor tile (every wait is
  * bounded at 2 s; such a launch writes statistics[0] = UINT64_MAX and later
@@ -421,9 +426,23 @@ int am_matmul_packed(uint32_t bits, const uint64_t* A, const uint64_t* B, uint32
                      uint32_t out_bits, uint64_t* C_packed, int32_t* C_i32, void* ws, uint64_t ws_bytes, void* stream);
 
 /* A batch of uneven products (descs in host memory) with one launch per
- * phase for the whole batch. */
+ * phase for the whole batch. The per-call header (job list, unpack and pack
+ * tables) is built on the host and copied into the workspace with a
+ * stream-ordered copy from pageable memory, so this call cannot be captured
+ * in a graph: use a plan for that. */
 int am_matmul_packed_batched(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws, uint64_t ws_bytes,
                              void* stream);
+
+/* A prepared batch: create validates the products, lays out `ws` and copies
+ * the header into it (stream-ordered on `stream`); run launches the kernels
+ * only (no host work, graph-capturable) and may be called any number of
+ * times while ws and the matrices stay in place; destroy frees the host
+ * record (no device memory is owned by a plan). */
+typedef struct am_matmul_plan am_matmul_plan;
+int am_matmul_plan_create(uint32_t bits, const am_matmul_desc* descs, uint32_t count, void* ws, uint64_t ws_bytes,
+                          am_matmul_plan** plan, void* stream);
+int am_matmul_plan_run(const am_matmul_plan* plan, void* stream);
+void am_matmul_plan_destroy(am_matmul_plan* plan);
 
 #ifdef __cplusplus
 }

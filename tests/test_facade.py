@@ -5,6 +5,8 @@ import json
 import os
 import subprocess
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -20,3 +22,19 @@ def test_facade_iterators(tmp_path):
     bits = out.stdout.splitlines()[0]
     assert [int(c) for c in bits] == g["index"]
     assert out.stdout.strip().endswith("OK")
+
+
+@pytest.mark.gpu
+def test_facade_wrappers_on_gpu():
+    """Every facade wrapper, called from C++ on the GPU, against the oracle
+    (tests/cpp/facade_gpu.cpp, prebuilt by __graft_entry__.build())."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    exe = os.path.join(ROOT, "tests", "cpp", "facade_gpu")
+    assert os.path.exists(exe), "tests/cpp/facade_gpu is not built (make -C tests/cpp)"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().endswith("ALL OK")
+    assert out.stdout.count("\nok ") + out.stdout.startswith("ok ") >= 16

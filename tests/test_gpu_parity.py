@@ -982,3 +982,55 @@ def test_pipeline_run_rejects_mismatched_launch(am, oracle_mod):
     torch.cuda.synchronize()
     outs, st = batch.results()
     _check_pipeline(oracle_mod, refs, outs, st, "after mismatches")
+
+
+def test_matmul_plan_and_graph_replay(am, oracle_mod):
+    """A prepared matmul batch (header uploaded once) run repeatedly and
+    replayed from a captured graph, every run checked against numpy int64
+    products; and the one-shot entry point on the same problems."""
+    import torch
+    rng = np.random.default_rng(61)
+    probs, exps = [], []
+    for (M, N, K) in ((300, 200, 500), (64, 700, 129), (513, 65, 1000)):
+        A = rng.integers(0, 128, (M, K)).astype(np.uint64)
+        B = rng.integers(0, 128, (K, N)).astype(np.uint64)
+        probs.append((torch.from_numpy(_pack_rows_u7(A).view(np.int64)).cuda(),
+                      torch.from_numpy(_pack_rows_u7(B).view(np.int64)).cuda(), M, N, K))
+        exps.append(A.astype(np.int64) @ B.astype(np.int64))
+    plan = am.MatmulPlan(7, probs, want_i32=True)
+
+    def check(tag):
+        for (cp, ci, ob), C, (_, _, M, N, K) in zip(plan.outs, exps, probs):
+            assert np.array_equal(ci.cpu().numpy().reshape(M, N).astype(np.int64), C), (tag, M, N, K)
+            assert np.array_equal(_unpack_rows(am.u64(cp), M, N, ob).astype(np.int64), C), (tag, M, N, K)
+
+    for rep in range(2):
+        for cp, ci, _ in plan.outs:
+            cp.zero_()
+            ci.zero_()
+        say("matmul plan run", rep)
+        plan.run()
+        torch.cuda.synchronize()
+        check(("run", rep))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.run()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run()
+    for rep in range(3):
+        for cp, ci, _ in plan.outs:
+            cp.zero_()
+            ci.zero_()
+        torch.cuda.synchronize()
+        say("matmul plan graph replay", rep)
+        g.replay()
+        torch.cuda.synchronize()
+        check(("replay", rep))
+    outs = am.matmul_packed_batched(7, probs, want_i32=True)
+    torch.cuda.synchronize()
+    for (cp, ci, ob), C, (_, _, M, N, K) in zip(outs, exps, probs):
+        assert np.array_equal(ci.cpu().numpy().reshape(M, N).astype(np.int64), C)
