@@ -194,7 +194,10 @@ extern "C" int am_plan_features(const am_pipeline_col* cols, uint32_t count, flo
         host[i].pr = make_pred(cols[i].pred, host[i].a.mask);
         host[i].c = cols[i].pred.c;
     }
-    hipError_t e = hipMemcpy(scratch_dev, host, count * sizeof(StatCol), hipMemcpyHostToDevice);
+    // stream-ordered: a previous call's k_plan_stats on this stream has read
+    // scratch_dev before it is overwritten (a copy from pageable memory
+    // returns once the source has been read, so `host` can go)
+    hipError_t e = hipMemcpyAsync(scratch_dev, host, count * sizeof(StatCol), hipMemcpyHostToDevice, (hipStream_t)stream);
     delete[] host;
     if (e != hipSuccess) return AM_E_HIP;
     hipLaunchKernelGGL(k_plan_stats, dim3(count), dim3(256), 0, (hipStream_t)stream, (const StatCol*)scratch_dev, count,

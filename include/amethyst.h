@@ -271,7 +271,8 @@ int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_out, void* 
  * written there too. out->n is the capacity in elements; writes beyond it
  * are dropped. stats_dev (device, 2 x uint64) receives
  * {selected count, number of selected values that did not fit out->bits};
- * values are stored masked to out->bits.
+ * values are stored masked to out->bits. The second count saturates at
+ * 2^29 - 1 (a planner only needs to know that values did not fit).
  * One launch, decoupled look-back: replaces kernel_compresstore_slabs_chunk_popcount
  * + cub::DeviceScan::ExclusiveSum + kernel_compresstore_slabs_write
  * (src/kernels/compressstore_slabs.cuh:4-206,
@@ -381,7 +382,7 @@ int am_pipeline_run(void* ws, uint64_t ws_bytes, uint32_t total_tiles, void* str
 
 /* ---------------------------------------------------------------- learned planner */
 
-/* 32 fp32 features per column of a batch (width, length, sampled
+/* 32 fp32 features per column of a batch (slab or bitstream inputs; width, length, sampled
  * selectivity, 16-bin histogram, sampled max of selected a+b, sortedness,
  * selection run length, byte estimates) into feat_dev[count][32]. scratch_dev
  * holds am_plan_scratch_bytes(count) bytes. No reference counterpart. */
