@@ -430,14 +430,14 @@ class PipelineBatch:
     """A prepared batch of fused filter -> compressstore(a+b) -> sum chains.
 
     columns: list of (a Column, b Column, (pred_op, c), s_fmt, s_bits).
-    variant: elements per lane per tile for every column (8 or 16, 0 = the
+    variant: elements per lane per tile for every column (8, 16 or 32, 0 = the
     launch-shape table's choice), or a list with one per column.
     run() is one launch over every tile of every column (capturable in a
     HIP graph); results: per column the output Column s (capacity n) and
     stats {count, overflow, total, 0}.
     """
 
-    def __init__(self, columns, variant=16, outs=None, stream=None):
+    def __init__(self, columns, variant=32, outs=None, stream=None):
         torch = _torch()
         n = len(columns)
         self.cols = columns

@@ -53,6 +53,30 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 template <typename T> __device__ inline void st_nt(T* p, T v) { __builtin_nontemporal_store(v, p); }
+template <typename T> __device__ inline void st_nt(__attribute__((address_space(1))) T* p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+// Pointers that kernels load from memory (job tables) are generic to the
+// compiler, whose flat accesses also count against the LDS wait counter, so
+// an LDS barrier would wait for every global load in flight. Device buffers
+// are global memory: these casts give global_load / global_store.
+#define AM_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ inline AM_GLOBAL T* gmem(T* p) { return (AM_GLOBAL T*)p; }
+template <typename T> __device__ inline const AM_GLOBAL T* gmem(const T* p) { return (const AM_GLOBAL T*)p; }
+// A load through the constant address space (scalar, kept in SGPRs): for
+// tables no kernel writes while it runs (job tables, recorded batch meta).
+template <typename T> __device__ inline T ldk(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)p;
+    T v;
+    uint32_t* d = (uint32_t*)&v;
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; i++) d[i] = q[i];
+    return v;
+}
 constexpr int LD_NT = 2; // cache-policy operand: non-temporal
 
 // 32-bit quotient idx/eps for idx < 2^24 and eps in 2..64 (multiply-high by
@@ -66,8 +90,8 @@ __device__ inline uint32_t div_eps(uint32_t idx, const Col& c)
 // formed once (scalar 64-bit math), per-element work is 32-bit.
 template <int KIND>
 struct TileReader {
-    const uint64_t* base; // first word touched by the tile
-    uint32_t r0;          // SLAB: t0 % eps ; BITSTREAM: (t0*bits) % 64
+    const AM_GLOBAL uint64_t* base; // first word touched by the tile
+    uint32_t r0;                    // SLAB: t0 % eps ; BITSTREAM: (t0*bits) % 64
     uint64_t mask;
     uint32_t bits, cbytes, eps, eps_magic;
     uint32_t last32; // BITSTREAM: last valid u32 index relative to base (clamp)
@@ -81,20 +105,20 @@ struct TileReader {
         eps = col.eps;
         eps_magic = col.eps_magic;
         if (KIND == K_STRIDED) {
-            base = (const uint64_t*)((const uint8_t*)col.p + t0 * col.cbytes);
+            base = (const AM_GLOBAL uint64_t*)((const AM_GLOBAL uint8_t*)gmem(col.p) + t0 * col.cbytes);
             r0 = 0;
         }
         else if (KIND == K_SLAB) {
             uint64_t q = t0 / col.eps;
-            base = col.p + q;
+            base = gmem(col.p) + q;
             r0 = (uint32_t)(t0 - q * col.eps);
         }
         else {
             uint64_t bit = t0 * (uint64_t)col.bits;
-            base = col.p + (bit >> 6);
+            base = gmem(col.p) + (bit >> 6);
             r0 = (uint32_t)(bit & 63);
         }
-        const uint64_t rel = col.words - (uint64_t)(base - col.p); // words from base to the end
+        const uint64_t rel = col.words - (uint64_t)(base - gmem(col.p)); // words from base to the end
         last32 = (uint32_t)(rel * 2 > 0xFFFFFFFFull ? 0xFFFFFFFFull : (rel ? rel * 2 - 1 : 0));
     }
 
@@ -103,9 +127,9 @@ struct TileReader {
     {
         if (KIND == K_STRIDED) {
             switch (cbytes) {
-                case 1: return ((const uint8_t*)base)[i] & mask;
-                case 2: return ((const uint16_t*)base)[i] & mask;
-                case 4: return ((const uint32_t*)base)[i] & mask;
+                case 1: return ((const AM_GLOBAL uint8_t*)base)[i] & mask;
+                case 2: return ((const AM_GLOBAL uint16_t*)base)[i] & mask;
+                case 4: return ((const AM_GLOBAL uint32_t*)base)[i] & mask;
                 default: return base[i] & mask;
             }
         }
@@ -118,7 +142,7 @@ struct TileReader {
         }
         else {
             uint32_t lb = r0 + i * bits;
-            const uint64_t* p = base + (lb >> 6);
+            const AM_GLOBAL uint64_t* p = base + (lb >> 6);
             uint32_t o = lb & 63;
             uint64_t v = p[0] >> o;
             if (o + bits > 64) v |= p[1] << (64 - o);
@@ -137,9 +161,9 @@ struct Fetch {
     {
         if (IK == K_STRIDED) {
             switch (rd.cbytes) { // uniform
-                case 1: raw = ((const uint8_t*)rd.base)[i]; break;
-                case 2: raw = ((const uint16_t*)rd.base)[i]; break;
-                case 4: raw = ((const uint32_t*)rd.base)[i]; break;
+                case 1: raw = ((const AM_GLOBAL uint8_t*)rd.base)[i]; break;
+                case 2: raw = ((const AM_GLOBAL uint16_t*)rd.base)[i]; break;
+                case 4: raw = ((const AM_GLOBAL uint32_t*)rd.base)[i]; break;
                 default: raw = rd.base[i];
             }
         }
@@ -150,7 +174,7 @@ struct Fetch {
         }
         else if (!WIDE) {
             const uint32_t lb = rd.r0 + i * rd.bits;
-            const uint32_t* p32 = (const uint32_t*)rd.base;
+            const AM_GLOBAL uint32_t* p32 = (const AM_GLOBAL uint32_t*)rd.base;
             const uint32_t i0 = lb >> 5;
             const uint32_t i1 = i0 + 1 <= rd.last32 ? i0 + 1 : rd.last32;
             raw = ((uint64_t)p32[i1] << 32) | p32[i0];
@@ -222,6 +246,32 @@ __device__ inline void lds_barrier()
     __builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0)
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
+}
+
+// Inclusive wave64 prefix sum with DPP: row_shr 1/2/4/8 within rows of 16
+// lanes (out-of-row sources read 0), then row_bcast 15 / 31 carry the row
+// totals into the following rows.
+template <int CTRL, int ROWM, bool BC>
+__device__ inline uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xf, BC);
+}
+__device__ inline uint32_t wave_incl_scan(uint32_t v)
+{
+    v += dpp_u32<0x111, 0xf, true>(v);  // row_shr:1
+    v += dpp_u32<0x112, 0xf, true>(v);  // row_shr:2
+    v += dpp_u32<0x114, 0xf, true>(v);  // row_shr:4
+    v += dpp_u32<0x118, 0xf, true>(v);  // row_shr:8
+    v += dpp_u32<0x142, 0xa, false>(v); // row_bcast:15 -> rows 1, 3
+    v += dpp_u32<0x143, 0xc, false>(v); // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
+__device__ inline uint32_t wave_sum32(uint32_t v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m);
+    return v;
 }
 
 // number of set bits of m below this lane (v_mbcnt_lo/hi)
@@ -806,7 +856,7 @@ __device__ inline void write_packed_tile(const Col& o, uint64_t ocap, const V* s
                     word |= v << sh;
                 }
             }
-            st_nt(o.p + kf + j, word);
+            st_nt(gmem(o.p) + kf + j, word);
         }
     }
     else {
@@ -839,7 +889,7 @@ __device__ inline void write_packed_tile(const Col& o, uint64_t ocap, const V* s
             }
             const uint64_t b0 = (kf + j) * 64;
             if (b0 + 64 > Bcap) word &= lowmask((uint32_t)(Bcap - b0));
-            st_nt(o.p + kf + j, word);
+            st_nt(gmem(o.p) + kf + j, word);
         }
     }
 }

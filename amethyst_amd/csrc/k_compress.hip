@@ -79,32 +79,6 @@ __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
     }
 }
 
-// Inclusive wave64 prefix sum with DPP: row_shr 1/2/4/8 within rows of 16
-// lanes (out-of-row sources read 0), then row_bcast 15 / 31 carry the row
-// totals into the following rows.
-template <int CTRL, int ROWM, bool BC>
-__device__ inline uint32_t dpp_u32(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xf, BC);
-}
-__device__ inline uint32_t wave_incl_scan(uint32_t v)
-{
-    v += dpp_u32<0x111, 0xf, true>(v);  // row_shr:1
-    v += dpp_u32<0x112, 0xf, true>(v);  // row_shr:2
-    v += dpp_u32<0x114, 0xf, true>(v);  // row_shr:4
-    v += dpp_u32<0x118, 0xf, true>(v);  // row_shr:8
-    v += dpp_u32<0x142, 0xa, false>(v); // row_bcast:15 -> rows 1, 3
-    v += dpp_u32<0x143, 0xc, false>(v); // row_bcast:31 -> rows 2, 3
-    return v;
-}
-
-__device__ inline uint32_t wave_sum32(uint32_t v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m);
-    return v;
-}
-
 __device__ inline uint64_t readlane64(uint64_t v, uint32_t j)
 {
     return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), j) << 32) |

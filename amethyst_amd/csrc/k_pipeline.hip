@@ -13,7 +13,7 @@
 //
 // Fused: compress(a)+compress(b) selects the same rows, so s = compress(a+b).
 // One block per tile, tiles numbered column by column; a column's tile size
-// (J elements per lane: 2048 or 4096 elements) is part of its plan. A block
+// (J elements per lane: 2048, 4096 or 8192 elements) is part of its plan. A block
 //   1. has each lane fetch its J elements of a and of b (element base +
 //      64j + lane; all loads in flight before any is used), evaluates the
 //      predicate on a and forms v = (a+b) & mask(w+1); the wave64 ballot of
@@ -39,7 +39,7 @@
 namespace am {
 
 constexpr uint32_t PL_WAVES = 4;
-constexpr uint32_t PL_JMAX = 16;
+constexpr uint32_t PL_JMAX = 32;
 __host__ __device__ constexpr uint32_t pl_tile(uint32_t J) { return PL_WAVES * J * 64; }
 constexpr uint32_t PL_TMAX = pl_tile(PL_JMAX);
 constexpr uint32_t PL_MAGIC = 0x41504C31u; // "APL1": the workspace holds a prepared batch
@@ -51,7 +51,7 @@ struct PipeJob {
     uint64_t* stats;  // device: {K, overflow, sum, 0}
     uint32_t tile0;   // first global tile
     uint32_t T;       // tiles of this column (>= 1)
-    uint32_t J;       // elements per lane per tile (8 or 16)
+    uint32_t J;       // elements per lane per tile (8, 16 or 32)
     uint32_t vbits;   // width of a+b (w+1)
     uint32_t chk_ovf; // s narrower than vbits
     uint32_t gbase;   // first look-back group of this column
@@ -62,6 +62,7 @@ struct PipeJob {
 struct PipeMeta {
     uint32_t magic, count, tiles, done; // done: columns finished in this launch
     uint64_t need;                      // workspace bytes the batch needs
+    uint64_t off_jobs;                  // byte offset of the job table
     uint64_t off_tiles, off_groups;     // byte offsets of the look-back areas
 };
 static_assert(sizeof(PipeMeta) <= sizeof(WsHeader) - 8, "meta fits the header");
@@ -69,6 +70,7 @@ __host__ __device__ inline PipeMeta* pl_meta(void* ws) { return (PipeMeta*)((uin
 
 struct PipeArgs {
     uint8_t* ws;
+    const uint32_t* map; // global tile -> column
     const PipeJob* jobs;
     WsHeader* hdr;
     PipeMeta* meta;
@@ -76,87 +78,136 @@ struct PipeArgs {
     uint64_t ws_bytes;
 };
 
-__device__ inline uint32_t pl_find(const PipeJob* jobs, uint32_t n, uint32_t tile)
-{
-    uint32_t lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile0 <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 // One tile of one column: decode, select, add, count and sum; compact the
 // selected values into LDS; look back within the column; write the output
 // words the tile completes. IK is the column's input kind (block-uniform).
-template <int IK, bool WIDEJ>
-__device__ inline void pl_tile_run(const PipeArgs& a, const PipeJob& J, uint32_t t, uint32_t epoch, uint32_t* s_vals,
-                                   uint32_t* s_wcnt, uint32_t* s_wovf, uint64_t* s_wsum, uint64_t* s_P,
-                                   uint64_t* s_carry)
-{
-    constexpr uint32_t F = LB_WIN | LB_OVF | LB_SUM;
-    constexpr uint32_t JJ = WIDEJ ? 16 : 8;
-    constexpr uint32_t TILE = pl_tile(JJ);
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t n = J.a.n;
-    const uint32_t bo = J.s.bits;
-    const uint64_t om = J.s.mask, vm = lowmask(J.vbits);
-    const uint64_t wbase = (uint64_t)t * TILE + wave * (JJ * 64);
-    const uint32_t wrem = wbase < n ? (uint32_t)(n - wbase < JJ * 64 ? n - wbase : JJ * 64) : 0;
-    const TileReader<IK> ra(J.a, wbase < n ? wbase : 0), rb(J.b, wbase < n ? wbase : 0);
-    auto idx = [&](uint32_t j) { return j * 64 + lane < wrem ? j * 64 + lane : (wrem ? wrem - 1 : 0); };
+// Tile totals of phase 1 (every thread holds them after the stage).
+struct TileTot {
+    uint32_t ctile, otile;
+    uint64_t stile;
+};
 
-    // ---- 1. select, add, count, sum (every load of the lane in flight first)
-    Fetch<IK, false> fa[JJ], fb[JJ];
-    if (wrem) {
+// Phase 1: the tile's J*256 elements in J/8 sub-passes of 2048 (8 per
+// lane); each lane fetches its elements of sub-pass sp+1 (all loads in
+// flight) before it uses those of sp, so a tile waits for one memory round
+// trip plus the sub-passes' arithmetic rather than one round trip each.
+// Per sub-pass the wave64 ballot of the predicate selects 64 consecutive
+// rows and the selected a+b go to s_vals at their tile ranks (sub-pass base
+// + the waves before + the rank in the ballots).
+constexpr uint32_t PL_SUB = 8; // elements per lane per sub-pass
+template <int IK>
+struct PlStage {
+    Fetch<IK, false> fa[PL_SUB], fb[PL_SUB];
+};
+template <int IK>
+__device__ inline TileTot pl_select_elems(const PipeJob& J, uint32_t t, uint32_t* s_vals, uint32_t* s_wcnt,
+                                          uint32_t* s_wovf, uint64_t* s_wsum)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint64_t n = J.a.n;
+    const uint64_t om = J.s.mask, vm = lowmask(J.vbits);
+    const uint32_t nsub = J.J / PL_SUB;
+    const uint64_t t0 = (uint64_t)t * pl_tile(J.J);
+    uint64_t vsum = 0;
+    uint32_t ovf = 0, base = 0;
+    // this wave's first element of sub-pass sp, and how many of its 512 exist
+    auto wbase = [&](uint32_t sp) { return t0 + sp * pl_tile(PL_SUB) + wave * (PL_SUB * 64); };
+    auto wrem = [&](uint64_t wb) -> uint32_t {
+        return wb < n ? (uint32_t)(n - wb < PL_SUB * 64 ? n - wb : PL_SUB * 64) : 0;
+    };
+    auto idx = [&](uint32_t j, uint32_t rem) { return j * 64 + lane < rem ? j * 64 + lane : (rem ? rem - 1 : 0); };
+    auto load = [&](PlStage<IK>& S, uint32_t sp) {
+        const uint64_t wb = wbase(sp);
+        const uint32_t rem = wrem(wb);
+        if (!rem) { // (past the column's end: nothing is selected)
 #pragma unroll
-        for (uint32_t j = 0; j < JJ; j++) {
-            fa[j].load(ra, idx(j));
-            fb[j].load(rb, idx(j));
+            for (uint32_t j = 0; j < PL_SUB; j++) S.fa[j].raw = S.fb[j].raw = 0;
+            return;
         }
-    }
-    uint64_t mine = 0, vsum = 0;
-    uint32_t cnt = 0, ovf = 0;
-    uint32_t val[JJ];
+        const TileReader<IK> ra(J.a, wb), rb(J.b, wb);
 #pragma unroll
-    for (uint32_t j = 0; j < JJ; j++) {
-        const uint64_t x = wrem ? fa[j].value(ra, idx(j)) : 0, y = wrem ? fb[j].value(rb, idx(j)) : 0;
-        const bool fl = (j * 64 + lane < wrem) && J.pr.sel32((uint32_t)x);
-        val[j] = (uint32_t)((x + y) & vm);
-        const uint64_t bl = ballot(fl);
-        if (lane == j) mine = bl;
-        cnt += (uint32_t)__popcll(bl);
-        vsum += fl ? val[j] : 0;
-        if (J.chk_ovf) ovf += (uint32_t)__popcll(ballot(fl && ((uint64_t)val[j] & ~om) != 0));
-    }
-    vsum = wave_sum_u64(vsum);
-    if (lane == 0) {
-        s_wcnt[wave] = cnt;
-        s_wovf[wave] = ovf;
-        s_wsum[wave] = vsum;
-    }
-    __syncthreads();
-    uint32_t wpre = 0, ctile = 0, otile = 0;
-    uint64_t stile = 0;
+        for (uint32_t j = 0; j < PL_SUB; j++) {
+            S.fa[j].load(ra, idx(j, rem));
+            S.fb[j].load(rb, idx(j, rem));
+        }
+    };
+    auto process = [&](const PlStage<IK>& S, uint32_t sp) {
+        const uint64_t wb = wbase(sp);
+        const uint32_t rem = wrem(wb);
+        const TileReader<IK> ra(J.a, rem ? wb : 0), rb(J.b, rem ? wb : 0);
+        uint64_t mine = 0;
+        uint32_t cnt = 0;
+        uint32_t val[PL_SUB];
 #pragma unroll
-    for (uint32_t w = 0; w < PL_WAVES; w++) {
-        wpre += w < wave ? s_wcnt[w] : 0;
-        ctile += s_wcnt[w];
-        otile += s_wovf[w];
-        stile += s_wsum[w];
-    }
-    {
-        uint32_t run = wpre;
+        for (uint32_t j = 0; j < PL_SUB; j++) {
+            const uint64_t x = S.fa[j].value(ra, idx(j, rem)), y = S.fb[j].value(rb, idx(j, rem));
+            const bool fl = (j * 64 + lane < rem) && J.pr.sel32((uint32_t)x);
+            val[j] = (uint32_t)((x + y) & vm);
+            const uint64_t bl = ballot(fl);
+            if (lane == j) mine = bl;
+            cnt += (uint32_t)__popcll(bl);
+            vsum += fl ? val[j] : 0;
+            // (an output at least as wide as a+b never overflows: val & ~om is 0)
+            ovf += (uint32_t)__popcll(ballot(fl && ((uint64_t)val[j] & ~om) != 0));
+        }
+        // (double-buffered counts: a wave reads sub-pass sp's before the
+        // barrier of sp+1, so sp+2 may overwrite them)
+        uint32_t* wc = s_wcnt + (sp & 1) * PL_WAVES;
+        if (lane == 0) wc[wave] = cnt;
+        lds_barrier();
+        uint32_t run = base, tot = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < JJ; j++) {
+        for (uint32_t w = 0; w < PL_WAVES; w++) {
+            run += w < wave ? wc[w] : 0;
+            tot += wc[w];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < PL_SUB; j++) {
             const uint64_t bl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
                                 (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
             if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = (uint32_t)(val[j] & om);
             run += (uint32_t)__popcll(bl);
         }
+        base += tot;
+    };
+    PlStage<IK> A, B;
+    load(A, 0);
+    for (uint32_t sp = 0; sp < nsub; sp += 2) {
+        if (sp + 1 < nsub) load(B, sp + 1);
+        process(A, sp);
+        if (sp + 1 < nsub) {
+            if (sp + 2 < nsub) load(A, sp + 2);
+            process(B, sp + 1);
+        }
+    }
+    vsum = wave_sum_u64(vsum);
+    if (lane == 0) {
+        s_wovf[wave] = ovf;
+        s_wsum[wave] = vsum;
     }
     __syncthreads();
+    uint32_t otile = 0;
+    uint64_t stile = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < PL_WAVES; w++) {
+        otile += s_wovf[w];
+        stile += s_wsum[w];
+    }
+    return TileTot{base, otile, stile};
+}
+
+// Phases 2 and 3 of a tile: look back within the column (wave 0), then
+// write the output words the tile completes.
+__device__ inline void pl_finish(const PipeArgs& a, const PipeMeta& m, const PipeJob& J, uint32_t t, uint32_t epoch,
+                                 const uint32_t* s_vals,
+                                 const TileTot& tt, uint64_t* s_P, uint64_t* s_carry)
+{
+    constexpr uint32_t F = LB_WIN | LB_OVF | LB_SUM;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t bo = J.s.bits;
+    const uint32_t ctile = tt.ctile, otile = tt.otile;
+    const uint64_t stile = tt.stile;
 
     // ---- 2. look-back within the column (wave 0)
     if (wave == 0) {
@@ -176,8 +227,8 @@ __device__ inline void pl_tile_run(const PipeArgs& a, const PipeJob& J, uint32_t
         L.epoch = epoch;
         L.bits = bo;
         L.hdr = a.hdr;
-        L.tiles = (uint64_t*)(a.ws + a.meta->off_tiles) + (uint64_t)J.tile0 * Lookback<F>::TS;
-        L.groups = (uint64_t*)(a.ws + a.meta->off_groups) + (uint64_t)J.gbase * Lookback<F>::GS;
+        L.tiles = (uint64_t*)(a.ws + m.off_tiles) + (uint64_t)J.tile0 * Lookback<F>::TS;
+        L.groups = (uint64_t*)(a.ws + m.off_groups) + (uint64_t)J.gbase * Lookback<F>::GS;
         const Agg own{ctile, own_w, otile, stile};
         const Agg ex = L.run(t, own, lane);
         if (lane == 0) {
@@ -185,14 +236,15 @@ __device__ inline void pl_tile_run(const PipeArgs& a, const PipeJob& J, uint32_t
             *s_carry = ex.w;
             if (t == J.T - 1) {
                 if (J.stats) {
-                    J.stats[0] = L.aborted() ? STATS_INVALID : ex.c + ctile;
-                    J.stats[1] = ovf_add(ex.o, otile);
-                    J.stats[2] = ex.s + stile;
-                    J.stats[3] = 0;
+                    AM_GLOBAL uint64_t* st = gmem(J.stats);
+                    st[0] = L.aborted() ? STATS_INVALID : ex.c + ctile;
+                    st[1] = ovf_add(ex.o, otile);
+                    st[2] = ex.s + stile;
+                    st[3] = 0;
                 }
                 // every tile of this column has published (so has read the
                 // epoch); the last column to get here advances it
-                if (add32(&a.meta->done, 1u) == a.meta->count - 1) {
+                if (add32(&a.meta->done, 1u) == m.count - 1) {
                     st32(&a.meta->done, 0u);
                     st32(&a.hdr->epoch, epoch + 1);
                 }
@@ -212,55 +264,55 @@ __device__ inline void pl_tile_run(const PipeArgs& a, const PipeJob& J, uint32_t
 __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
 {
     __shared__ uint32_t s_vals[PL_TMAX]; // a+b of <= 31-bit inputs fits 32 bits
-    __shared__ uint32_t s_wcnt[PL_WAVES], s_wovf[PL_WAVES];
+    __shared__ uint32_t s_wcnt[2 * PL_WAVES], s_wovf[PL_WAVES];
     __shared__ uint64_t s_wsum[PL_WAVES];
     __shared__ uint64_t s_P, s_carry;
     __shared__ uint32_t s_epoch, s_abort;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t g = blockIdx.x; // global tile (one per block, in order)
-    // ---- the launch must match the prepared batch
-    {
-        const PipeMeta m = *a.meta;
-        if (m.magic != PL_MAGIC) return; // never prepared: nothing to run or report
-        if (m.tiles != a.total_tiles || m.need > a.ws_bytes) {
-            if (g == 0)
-                for (uint32_t c = tid; c < m.count; c += 256)
-                    if (a.jobs[c].stats) a.jobs[c].stats[0] = STATS_INVALID;
-            return;
-        }
+    // ---- the launch must match the prepared batch (the tile map lies
+    // within the workspace whatever it holds: read with the meta at once)
+    const PipeMeta m = ldk(a.meta);
+    const uint32_t ji = ldk(a.map + g);
+    if (m.magic != PL_MAGIC) return; // never prepared: nothing to run or report
+    if (m.tiles != a.total_tiles || m.need > a.ws_bytes || ji >= m.count) {
+        // (the table as prepared -- its place depends on the prepared tile
+        // count -- within the extent the prepare checked)
+        const PipeJob* jobs = (const PipeJob*)(a.ws + m.off_jobs);
+        if (g == 0)
+            for (uint32_t c = tid; c < m.count; c += 256)
+                if (jobs[c].stats) jobs[c].stats[0] = STATS_INVALID;
+        return;
     }
-    const PipeJob& J = a.jobs[pl_find(a.jobs, a.meta->count, g)];
-    const uint32_t t = g - J.tile0;
     if (tid == 0) {
         s_epoch = ld32(&a.hdr->epoch);
         s_abort = ld32(&a.hdr->abort);
     }
+    const PipeJob J = ldk(a.jobs + ji); // (uniform: scalar registers for the whole tile)
+    const uint32_t t = g - J.tile0;
     __syncthreads();
     if (s_abort) {
-        if (t == J.T - 1 && tid == 0 && J.stats) J.stats[0] = STATS_INVALID;
+        if (t == J.T - 1 && tid == 0 && J.stats) gmem(J.stats)[0] = STATS_INVALID;
         return;
     }
-#define PL_RUN(IK)                                                                                             \
-    {                                                                                                          \
-        if (J.J == 16) pl_tile_run<IK, true>(a, J, t, s_epoch, s_vals, s_wcnt, s_wovf, s_wsum, &s_P, &s_carry);  \
-        else pl_tile_run<IK, false>(a, J, t, s_epoch, s_vals, s_wcnt, s_wovf, s_wsum, &s_P, &s_carry);          \
-    }
+    TileTot tt;
     switch (J.a.kind) { // (one kind per column; in practice one per batch)
-        case K_SLAB: PL_RUN(K_SLAB); break;
-        case K_BITSTREAM: PL_RUN(K_BITSTREAM); break;
-        default: PL_RUN(K_STRIDED); break;
+        case K_SLAB: tt = pl_select_elems<K_SLAB>(J, t, s_vals, s_wcnt, s_wovf, s_wsum); break;
+        case K_BITSTREAM: tt = pl_select_elems<K_BITSTREAM>(J, t, s_vals, s_wcnt, s_wovf, s_wsum); break;
+        default: tt = pl_select_elems<K_STRIDED>(J, t, s_vals, s_wcnt, s_wovf, s_wsum); break;
     }
-#undef PL_RUN
+    pl_finish(a, m, J, t, s_epoch, s_vals, tt, &s_P, &s_carry);
 }
 
 } // namespace am
 
 using namespace am;
 
-// Workspace: header (epoch, abort, the batch's meta) + small region, job
-// table, then the look-back area: tile descriptors for every tile of the
-// batch and (tiles/64 + 1) group descriptors per column.
+// Workspace: header (epoch, abort, the batch's meta) + small region, the
+// tile map (global tile -> column, u32), the job table, then the look-back
+// area: tile descriptors for every tile of the batch and (tiles/64 + 1)
+// group descriptors per column.
 using PlLookback = Lookback<LB_WIN | LB_OVF | LB_SUM>;
 
 static uint64_t pl_lb_bytes(uint64_t tiles, uint64_t groups)
@@ -269,6 +321,8 @@ static uint64_t pl_lb_bytes(uint64_t tiles, uint64_t groups)
 }
 
 static uint64_t pl_jobs_bytes(uint32_t count) { return ((uint64_t)count * sizeof(PipeJob) + 255) / 256 * 256; }
+static uint64_t pl_map_bytes(uint64_t tiles) { return (tiles * 4 + 255) / 256 * 256; }
+constexpr uint64_t PL_OFF_MAP = 256 + WS_SMALL_BYTES;
 
 extern "C" uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t count)
 {
@@ -279,7 +333,7 @@ extern "C" uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t coun
         tiles += t;
         groups += t / 64 + 1;
     }
-    return 256 + WS_SMALL_BYTES + pl_jobs_bytes(count) + pl_lb_bytes(tiles, groups);
+    return PL_OFF_MAP + pl_map_bytes(tiles) + pl_jobs_bytes(count) + pl_lb_bytes(tiles, groups);
 }
 
 // Host-side validation + job table upload; after this, am_pipeline_run
@@ -305,9 +359,10 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
         uint32_t v = c.variant;
         if (v == 0) {
             v = am_tuned(AM_KNOB_PIPELINE_TILE, c.a.bits, c.a.format, c.s.format);
-            if (v == 0) v = 16;
+            if (v == 0) v = 32;
         }
-        if (v != 8 && v != 16) return AM_E_INVALID;
+        if (v != 8 && v != 16 && v != 32) return AM_E_INVALID;
+        static_assert(PL_JMAX == 32, "the accepted tile sizes fit s_vals");
         J.J = v;
         J.pr = make_pred(c.pred, J.a.mask);
         J.stats = c.stats;
@@ -321,14 +376,18 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
         groups += J.T / 64 + 1;
         if (tiles >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
     }
-    const uint64_t need = 256 + WS_SMALL_BYTES + pl_jobs_bytes(count) + pl_lb_bytes(tiles, groups);
+    const uint64_t off_j = PL_OFF_MAP + pl_map_bytes(tiles), off_t = off_j + pl_jobs_bytes(count);
+    const uint64_t need = off_t + pl_lb_bytes(tiles, groups);
     if (ws_bytes < need) return AM_E_WORKSPACE;
+    std::vector<uint32_t> map(tiles);
+    for (uint32_t i = 0; i < count; i++) std::fill(map.begin() + jobs[i].tile0, map.begin() + jobs[i].tile0 + jobs[i].T, i);
     // the previous run on this stream may still read the job table
     if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return AM_E_HIP;
-    uint8_t* jt = (uint8_t*)ws + 256 + WS_SMALL_BYTES;
-    if (hipMemcpy(jt, jobs.data(), count * sizeof(PipeJob), hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
-    const uint64_t off_t = 256 + WS_SMALL_BYTES + pl_jobs_bytes(count);
-    PipeMeta m{PL_MAGIC, count, (uint32_t)tiles, 0, need, off_t, off_t + tiles * PlLookback::TS * 8};
+    uint8_t* w8 = (uint8_t*)ws;
+    if (hipMemcpy(w8 + PL_OFF_MAP, map.data(), tiles * 4, hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
+    if (hipMemcpy(w8 + off_j, jobs.data(), count * sizeof(PipeJob), hipMemcpyHostToDevice) != hipSuccess)
+        return AM_E_HIP;
+    PipeMeta m{PL_MAGIC, count, (uint32_t)tiles, 0, need, off_j, off_t, off_t + tiles * PlLookback::TS * 8};
     if (hipMemcpy(pl_meta(ws), &m, sizeof(m), hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
     if (total_tiles) *total_tiles = (uint32_t)tiles;
     return AM_OK;
@@ -336,8 +395,11 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
 
 extern "C" int am_pipeline_run(void* ws, uint64_t ws_bytes, uint32_t total_tiles, void* stream)
 {
-    if (!ws || ws_bytes < 256 + WS_SMALL_BYTES) return AM_E_WORKSPACE;
+    if (!ws || ws_bytes < PL_OFF_MAP) return AM_E_WORKSPACE;
     if (total_tiles == 0) return AM_E_INVALID;
+    // every block reads its tile-map entry before it checks the recorded
+    // batch, so the map alone must fit whatever the workspace holds
+    if (ws_bytes < PL_OFF_MAP + pl_map_bytes(total_tiles)) return AM_E_WORKSPACE;
     // everything else -- job table, look-back areas, the tile count the grid
     // must have -- was recorded by am_pipeline_prepare in the workspace and
     // is checked by the kernel itself (the launch stays graph-capturable)
@@ -345,7 +407,8 @@ extern "C" int am_pipeline_run(void* ws, uint64_t ws_bytes, uint32_t total_tiles
     a.ws = (uint8_t*)ws;
     a.hdr = (WsHeader*)ws;
     a.meta = pl_meta(ws);
-    a.jobs = (const PipeJob*)((uint8_t*)ws + 256 + WS_SMALL_BYTES);
+    a.map = (const uint32_t*)((uint8_t*)ws + PL_OFF_MAP);
+    a.jobs = (const PipeJob*)((uint8_t*)ws + PL_OFF_MAP + pl_map_bytes(total_tiles));
     a.total_tiles = total_tiles;
     a.ws_bytes = ws_bytes;
     hipLaunchKernelGGL(k_pipeline, dim3(total_tiles), dim3(256), 0, (hipStream_t)stream, a);

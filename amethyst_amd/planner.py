@@ -5,14 +5,14 @@ Plan space, per column (none of it changes any integer arithmetic):
   * output width: w+1 (the add rule, always safe), or the sampled maximum
     width "est" / "est+1" (narrower, fewer bytes; a value that does not fit
     is detected on the device and the column re-runs at w+1);
-  * kernel variant: 2048- or 4096-element tiles (J = 8 or 16), per column
+  * kernel variant: 8192- or 4096-element tiles (J = 32 or 16), per column
     within the batch's one launch.
 
 The planner is an MLP (32 -> 64 -> 64 -> 12) over per-column statistics that
 the HIP kernel k_plan_stats computes; it predicts each plan's log-time
 relative to the best plan and runs as the hand-written HIP kernel
 k_plan_mlp. When the predicted best beats the runner-up by less than a
-margin, the column takes the analytic plan (bitstream, width w+1, J = 16:
+margin, the column takes the analytic plan (bitstream, width w+1, J = 32:
 fewest bytes among the always-safe plans). Training data are the timings of
 every plan on seeded columns, produced by this module's own sweep
 (`collect`); `evaluate` measures regret against the exhaustively best plan
@@ -37,9 +37,9 @@ from . import (BITSTREAM, LT, SLAB, Column, PipelineBatch, _check, _stream, devi
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "planner_weights.npz")
 
-PLANS = [(fmt, wr, var) for var in (16, 8) for fmt in (SLAB, BITSTREAM) for wr in ("w+1", "est", "est+1")]
+PLANS = [(fmt, wr, var) for var in (32, 16) for fmt in (SLAB, BITSTREAM) for wr in ("w+1", "est", "est+1")]
 NPLANS = len(PLANS)
-ANALYTIC = PLANS.index((BITSTREAM, "w+1", 16))
+ANALYTIC = PLANS.index((BITSTREAM, "w+1", 32))
 NFEAT, NHID = 32, 64
 
 
@@ -141,7 +141,7 @@ def run_batch(cols, plans, feats, reps=0):
     redo = [i for i in range(len(cols)) if int(stats[i][1])]
     if redo:  # verified fallback: values wider than the chosen width
         spec = [(cols[i][0], cols[i][1], cols[i][2], PLANS[plans[i]][0], cols[i][0].bits + 1) for i in redo]
-        bt2 = PipelineBatch(spec, variant=16)
+        bt2 = PipelineBatch(spec, variant=32)
         bt2.run()
         torch.cuda.synchronize()
         o2, st2 = bt2.results()

@@ -234,7 +234,7 @@ typedef enum {
     AM_KNOB_BINARY_WORDS = 2,    /* binary ops on slab/container words: output words per thread, 2, 4 or 8 */
     AM_KNOB_SUM_BLOCKS = 3,      /* sum: blocks of the partial-sum launch, 64..8192 */
     AM_KNOB_GROUPBY_BLOCKS = 4,  /* group-by over <= 12-bit keys: row blocks, 64..2048 */
-    AM_KNOB_PIPELINE_TILE = 5,   /* batched chain: elements per lane per tile (8 or 16) for variant 0 */
+    AM_KNOB_PIPELINE_TILE = 5,   /* batched chain: elements per lane per tile (8, 16 or 32) for variant 0 */
     AM_KNOB_COUNT = 6
 } am_knob;
 
@@ -349,9 +349,9 @@ int am_groupby(const am_column* keys, const am_column* values, const am_predicat
  * bitstream), width and capacity (s.n); values wider than s.bits are stored
  * masked and counted. stats (device, 4 x uint64) receives {selected count,
  * values that did not fit s.bits, total, 0}. a and b: same n, bits (<= 31)
- * and format (any format). variant: elements per lane per tile, 8 or 16
- * (2048- or 4096-element tiles), or 0 for the launch-shape table's choice
- * (AM_KNOB_PIPELINE_TILE), else 16. */
+ * and format (any format). variant: elements per lane per tile, 8, 16 or
+ * 32 (2048-, 4096- or 8192-element tiles), or 0 for the launch-shape
+ * table's choice (AM_KNOB_PIPELINE_TILE), else 32. */
 typedef struct {
     am_column a;
     am_column b;

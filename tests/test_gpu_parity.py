@@ -832,10 +832,10 @@ def _check_pipeline(oracle_mod, refs, outs, st, tag):
 
 
 @pytest.mark.parametrize("fmt_in", [1, 2, 4, 3])
-@pytest.mark.parametrize("variant", [8, 16])
+@pytest.mark.parametrize("variant", [8, 16, 32])
 def test_pipeline_variants_and_formats(am, oracle_mod, fmt_in, variant):
     """The fused chain for every input decoder (slab, bitstream, u32, tight)
-    and both tile sizes: packed s words, counts, overflow and totals."""
+    and every tile size: packed s words, counts, overflow and totals."""
     import torch
     rng = np.random.default_rng(100 + 10 * fmt_in + variant)
     cols, refs = _pipeline_cols(am, oracle_mod, rng, 14, fmt_in)
@@ -933,13 +933,13 @@ def test_groupby_wide_keys(am, oracle_mod):
 
 
 def test_pipeline_mixed_tiles_one_launch(am, oracle_mod):
-    """Columns of both tile sizes and several input formats' worth of widths
+    """Columns of every tile size and several input formats' worth of widths
     in ONE launch, run repeatedly on one workspace; then a second batch
     prepared on the same workspace."""
     import torch
     rng = np.random.default_rng(91)
     cols, refs = _pipeline_cols(am, oracle_mod, rng, 18, 2)
-    variants = [(8, 16, 0)[i % 3] for i in range(len(cols))]
+    variants = [(8, 16, 32, 0)[i % 4] for i in range(len(cols))]
     batch = am.PipelineBatch(cols, variant=variants)
     for _ in range(3):
         batch.run()
@@ -969,8 +969,11 @@ def test_pipeline_run_rejects_mismatched_launch(am, oracle_mod):
     rng = np.random.default_rng(92)
     cols, refs = _pipeline_cols(am, oracle_mod, rng, 5, 1)
     batch = am.PipelineBatch(cols, variant=16)
+    map_end = 256 + 65536 + (4 * batch.total_tiles + 255) // 256 * 256  # header, small region, tile map
+    assert am.lib().am_pipeline_run(am.ctypes.c_void_p(batch.ws.data_ptr()), map_end - 8, batch.total_tiles,
+                                    am._stream()) == 4  # AM_E_WORKSPACE: the map itself does not fit
     for tiles, wsb in ((batch.total_tiles + 1, batch.ws_bytes), (max(1, batch.total_tiles - 1), batch.ws_bytes),
-                       (batch.total_tiles, 256 + 65536 + 64)):  # less than the batch's job table and descriptors
+                       (batch.total_tiles, map_end)):  # less than the batch's job table and descriptors
         batch.stats.zero_()
         st = am.lib().am_pipeline_run(am.ctypes.c_void_p(batch.ws.data_ptr()), wsb, tiles, am._stream())
         assert st == 0
