@@ -51,11 +51,14 @@ struct PipeJob {
     uint64_t* stats;  // device: {K, overflow, sum, 0}
     uint32_t tile0;   // first global tile
     uint32_t T;       // tiles of this column (>= 1)
-    uint32_t J;       // elements per lane per tile (8, 16 or 32)
+    uint32_t J;       // element path: elements per lane per tile (8..128)
     uint32_t vbits;   // width of a+b (w+1)
     uint32_t chk_ovf; // s narrower than vbits
     uint32_t gbase;   // first look-back group of this column
-    uint32_t pad0, pad1;
+    uint32_t wsub;    // word path (slab a, b of 2..31 bits): 256-word sub-passes per tile; 0: element path
+    uint32_t vsz;     // bytes per staged value: 1, 2 or 4 (s.bits rounded up)
+    TopPred tp;       // word path: the predicate over top-aligned fields
+    uint32_t pad1;
 };
 
 // Recorded by prepare in the workspace header's spare words.
@@ -99,8 +102,8 @@ template <int IK>
 struct PlStage {
     Fetch<IK, false> fa[PL_SUB], fb[PL_SUB];
 };
-template <int IK>
-__device__ inline TileTot pl_select_elems(const PipeJob& J, uint32_t t, uint32_t* s_vals, uint32_t* s_wcnt,
+template <int IK, typename V>
+__device__ inline TileTot pl_select_elems(const PipeJob& J, uint32_t t, V* s_vals, uint32_t* s_wcnt,
                                           uint32_t* s_wovf, uint64_t* s_wsum)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
@@ -166,7 +169,7 @@ __device__ inline TileTot pl_select_elems(const PipeJob& J, uint32_t t, uint32_t
         for (uint32_t j = 0; j < PL_SUB; j++) {
             const uint64_t bl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(mine >> 32), j) << 32) |
                                 (uint32_t)__builtin_amdgcn_readlane((uint32_t)mine, j);
-            if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = (uint32_t)(val[j] & om);
+            if ((bl >> lane) & 1) s_vals[run + mbcnt(bl)] = (V)(val[j] & om);
             run += (uint32_t)__popcll(bl);
         }
         base += tot;
@@ -197,10 +200,100 @@ __device__ inline TileTot pl_select_elems(const PipeJob& J, uint32_t t, uint32_t
     return TileTot{base, otile, stile};
 }
 
+// Phase 1 of the word path (slab a and b of 2..31 bits, eps = 64/bits
+// fields per word): the tile is J.wsub sub-passes of 256 words, lane
+// `tid` takes word tid of each (so a wave's words, and their rows, are
+// consecutive); the words of the next four sub-passes are in flight while
+// one is used. Per word: the selection bits of its eps fields at once
+// (select_fields: one compare and one add-with-carry per field), the rank
+// of the lane's first selected row by a DPP scan of the per-lane counts
+// plus the waves before, then a+b of each selected field to s_vals.
+constexpr uint32_t PL_RING = 4;
+template <typename V>
+__device__ inline TileTot pl_select_words(const PipeJob& J, uint32_t t, V* s_vals, uint32_t* s_wcnt,
+                                          uint32_t* s_wovf, uint64_t* s_wsum)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t w = J.a.bits, eps = J.a.eps, nsub = J.wsub;
+    const uint32_t m = (uint32_t)J.a.mask;
+    const uint32_t om = J.s.bits >= 32 ? 0xFFFFFFFFu : (uint32_t)J.s.mask;
+    const uint64_t words = J.a.words, n = J.a.n;
+    const uint64_t tw0 = (uint64_t)t * nsub * 256; // first word of the tile
+    const AM_GLOBAL uint64_t* pa = gmem(J.a.p);
+    const AM_GLOBAL uint64_t* pb = gmem(J.b.p);
+    // the tile reaches past the column's last element: mask fields >= n
+    const bool tail = (tw0 + (uint64_t)nsub * 256) * eps > n;
+    uint64_t X[PL_RING], Y[PL_RING];
+    auto fetch = [&](uint32_t k, uint32_t sp) {
+        const uint64_t wi = tw0 + sp * 256 + tid;
+        X[k] = wi < words ? pa[wi] : 0;
+        Y[k] = wi < words ? pb[wi] : 0;
+    };
+#pragma unroll
+    for (uint32_t k = 0; k < PL_RING; k++)
+        if (k < nsub) fetch(k, k);
+    uint64_t vsum = 0;
+    uint32_t ovf = 0, base = 0;
+    for (uint32_t sp0 = 0; sp0 < nsub; sp0 += PL_RING) {
+#pragma unroll
+        for (uint32_t k = 0; k < PL_RING; k++) {
+            const uint32_t sp = sp0 + k;
+            if (sp >= nsub) break;
+            const uint64_t x = X[k], y = Y[k];
+            if (sp + PL_RING < nsub) fetch(k, sp + PL_RING);
+            uint32_t gg = select_fields(x, eps, w, w, J.tp);
+            if (tail) {
+                const uint64_t e0 = (tw0 + sp * 256 + tid) * eps;
+                const uint64_t left = e0 < n ? n - e0 : 0;
+                gg &= left >= eps ? 0xFFFFFFFFu : (uint32_t)((1ull << left) - 1);
+            }
+            const uint32_t c = (uint32_t)__popc(gg);
+            const uint32_t incl = wave_incl_scan(c);
+            // (double-buffered counts: a wave reads sub-pass sp's before the
+            // barrier of sp+1, so sp+2 may overwrite them)
+            uint32_t* wc = s_wcnt + (sp & 1) * PL_WAVES;
+            if (lane == 63) wc[wave] = incl;
+            lds_barrier();
+            uint32_t pos = base + incl - c, tot = 0;
+#pragma unroll
+            for (uint32_t v = 0; v < PL_WAVES; v++) {
+                pos += v < wave ? wc[v] : 0;
+                tot += wc[v];
+            }
+            for (uint32_t f = 0, sh = 0; f < eps; f++, sh += w) {
+                const uint32_t v = ((uint32_t)(x >> sh) & m) + ((uint32_t)(y >> sh) & m); // < 2^32
+                const bool sel = (gg >> f) & 1;
+                if (sel) s_vals[pos] = (V)(v & om);
+                pos += sel;
+                vsum += sel ? v : 0;
+                ovf += (sel && (v & ~om) != 0) ? 1 : 0; // (0 when s holds w+1 bits)
+            }
+            base += tot;
+        }
+    }
+    vsum = wave_sum_u64(vsum);
+    ovf = wave_sum32(ovf);
+    if (lane == 0) {
+        s_wovf[wave] = ovf;
+        s_wsum[wave] = vsum;
+    }
+    __syncthreads();
+    uint32_t otile = 0;
+    uint64_t stile = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < PL_WAVES; v++) {
+        otile += s_wovf[v];
+        stile += s_wsum[v];
+    }
+    return TileTot{base, otile, stile};
+}
+
 // Phases 2 and 3 of a tile: look back within the column (wave 0), then
 // write the output words the tile completes.
+template <typename V>
 __device__ inline void pl_finish(const PipeArgs& a, const PipeMeta& m, const PipeJob& J, uint32_t t, uint32_t epoch,
-                                 const uint32_t* s_vals,
+                                 const V* s_vals,
                                  const TileTot& tt, uint64_t* s_P, uint64_t* s_carry)
 {
     constexpr uint32_t F = LB_WIN | LB_OVF | LB_SUM;
@@ -296,13 +389,21 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
         if (t == J.T - 1 && tid == 0 && J.stats) gmem(J.stats)[0] = STATS_INVALID;
         return;
     }
-    TileTot tt;
-    switch (J.a.kind) { // (one kind per column; in practice one per batch)
-        case K_SLAB: tt = pl_select_elems<K_SLAB>(J, t, s_vals, s_wcnt, s_wovf, s_wsum); break;
-        case K_BITSTREAM: tt = pl_select_elems<K_BITSTREAM>(J, t, s_vals, s_wcnt, s_wovf, s_wsum); break;
-        default: tt = pl_select_elems<K_STRIDED>(J, t, s_vals, s_wcnt, s_wovf, s_wsum); break;
-    }
-    pl_finish(a, m, J, t, s_epoch, s_vals, tt, &s_P, &s_carry);
+    // the compacted values are staged at 1, 2 or 4 bytes (the output width
+    // rounded up), so narrow outputs get larger tiles in the same LDS
+    auto tile = [&](auto* sv) {
+        TileTot tt;
+        if (J.wsub) tt = pl_select_words(J, t, sv, s_wcnt, s_wovf, s_wsum);
+        else switch (J.a.kind) { // (one kind per column; in practice one per batch)
+            case K_SLAB: tt = pl_select_elems<K_SLAB>(J, t, sv, s_wcnt, s_wovf, s_wsum); break;
+            case K_BITSTREAM: tt = pl_select_elems<K_BITSTREAM>(J, t, sv, s_wcnt, s_wovf, s_wsum); break;
+            default: tt = pl_select_elems<K_STRIDED>(J, t, sv, s_wcnt, s_wovf, s_wsum); break;
+        }
+        pl_finish(a, m, J, t, s_epoch, sv, tt, &s_P, &s_carry);
+    };
+    if (J.vsz == 1) tile((uint8_t*)s_vals);
+    else if (J.vsz == 2) tile((uint16_t*)s_vals);
+    else tile(s_vals);
 }
 
 } // namespace am
@@ -362,14 +463,31 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
             if (v == 0) v = 32;
         }
         if (v != 8 && v != 16 && v != 32) return AM_E_INVALID;
+        // staged values: the output width rounded up to 1, 2 or 4 bytes; the
+        // tile grows by 4/vsz (s_vals holds PL_TMAX u32 = 4*PL_TMAX bytes)
+        J.vsz = c.s.bits <= 8 ? 1 : c.s.bits <= 16 ? 2 : 4;
+        const uint32_t f = 4 / J.vsz, vf = v * f;
         static_assert(PL_JMAX == 32, "the accepted tile sizes fit s_vals");
-        J.J = v;
+        J.J = vf;
         J.pr = make_pred(c.pred, J.a.mask);
         J.stats = c.stats;
         J.vbits = c.a.bits + 1;
         J.chk_ovf = c.s.bits < J.vbits ? 1 : 0;
-        const uint32_t TL = pl_tile(v);
-        J.T = J.a.n ? (uint32_t)((J.a.n + TL - 1) / TL) : 1;
+        // word path: whole slab words per tile, about J*256 elements: at
+        // least 2048 (the tile count am_pipeline_workspace_bytes allows for)
+        // and at most PL_TMAX (s_vals)
+        J.wsub = (J.a.kind == K_SLAB && c.a.bits >= 2)
+                     ? std::min(std::max(vf / J.a.eps, (8 + J.a.eps - 1) / J.a.eps), PL_JMAX * f / J.a.eps)
+                     : 0;
+        J.tp = J.wsub ? make_top_pred(J.pr, c.a.bits, J.a.eps) : TopPred{0, 0, 0};
+        if (J.wsub) {
+            const uint64_t TW = (uint64_t)J.wsub * 256;
+            J.T = J.a.words ? (uint32_t)((J.a.words + TW - 1) / TW) : 1;
+        }
+        else {
+            const uint32_t TL = pl_tile(vf);
+            J.T = J.a.n ? (uint32_t)((J.a.n + TL - 1) / TL) : 1;
+        }
         J.tile0 = (uint32_t)tiles;
         J.gbase = (uint32_t)groups;
         tiles += J.T;

@@ -848,6 +848,38 @@ def test_pipeline_variants_and_formats(am, oracle_mod, fmt_in, variant):
     _check_pipeline(oracle_mod, refs, outs, st, (fmt_in, variant))
 
 
+def test_pipeline_slab_every_width_and_predicate(am, oracle_mod):
+    """Slab inputs of every width 1..31 (2..31 take the word-per-lane
+    decode, 1 the per-element one) with all six predicates, every tile size,
+    lengths that end mid-word and mid-tile, values at the edges of the
+    range, in one batch: packed s words, counts, overflow and totals."""
+    import torch
+    rng = np.random.default_rng(2024)
+    ops = {0: np.less, 1: np.less_equal, 2: np.greater, 3: np.greater_equal, 4: np.equal, 5: np.not_equal}
+    cols, refs, variants = [], [], []
+    for w in range(1, 32):
+        n = int(rng.integers(1, 70000)) if w % 5 else int(rng.integers(8192 * 3, 8192 * 5))
+        a = rand_vals(rng, n, w)
+        a[rng.random(n) < 0.05] = (1 << w) - 1  # the top value and zero are common predicate edges
+        a[rng.random(n) < 0.05] = 0
+        b = rand_vals(rng, n, w)
+        op = int(rng.integers(0, 6))
+        c = int(rng.integers(0, 1 << w)) if op < 4 else int(a[rng.integers(0, n)])
+        fmt = (1, 2)[w % 2]
+        sb = w + 1 if w % 3 else max(1, w - 1)
+        cols.append((upload(am, oracle_mod, a, w, 1), upload(am, oracle_mod, b, w, 1), (op, c), fmt, sb))
+        v = (a + b) & np.uint64((1 << (w + 1)) - 1)
+        refs.append((v, ops[op](a, np.uint64(c)), fmt, sb))
+        variants.append((8, 16, 32)[w % 3])
+    say("pipeline slab widths 1..31")
+    batch = am.PipelineBatch(cols, variant=variants)
+    batch.run()
+    batch.run()
+    torch.cuda.synchronize()
+    outs, st = batch.results()
+    _check_pipeline(oracle_mod, refs, outs, st, "slab widths")
+
+
 def test_pipeline_graph_replay(am, oracle_mod):
     """Config C4's launch form: the batch captured once in a graph and
     replayed; before every replay the outputs and statistics are cleared
