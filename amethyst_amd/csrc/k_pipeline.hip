@@ -39,9 +39,11 @@
 namespace am {
 
 constexpr uint32_t PL_WAVES = 4;
-constexpr uint32_t PL_JMAX = 32;
 __host__ __device__ constexpr uint32_t pl_tile(uint32_t J) { return PL_WAVES * J * 64; }
-constexpr uint32_t PL_TMAX = pl_tile(PL_JMAX);
+// LDS staging of a tile's compacted values, in bytes: with the block's other
+// shared words just under 40 KB, so four blocks fill a CU's 160 KB
+constexpr uint32_t PL_STAGE_BYTES = 40448;
+constexpr uint32_t PL_TMAX = PL_STAGE_BYTES / 4; // u32 values
 constexpr uint32_t PL_MAGIC = 0x41504C31u; // "APL1": the workspace holds a prepared batch
 
 struct PipeJob {
@@ -61,9 +63,19 @@ struct PipeJob {
     uint32_t pad1;
 };
 
+// A block's work: column `job`, tile t of it (job PL_IDLE: a padding block).
+struct PlSlot {
+    uint32_t job, t;
+};
+constexpr uint32_t PL_IDLE = 0xFFFFFFFFu;
+// The GPU's XCDs (MI355X: 8, each with its own L2); workgroups go to them
+// round-robin by index, in order within each.
+constexpr uint32_t PL_XCDS = 8;
+
 // Recorded by prepare in the workspace header's spare words.
 struct PipeMeta {
     uint32_t magic, count, tiles, done; // done: columns finished in this launch
+    uint32_t grid, xcd_local;           // blocks of a run; columns kept on one XCD each
     uint64_t need;                      // workspace bytes the batch needs
     uint64_t off_jobs;                  // byte offset of the job table
     uint64_t off_tiles, off_groups;     // byte offsets of the look-back areas
@@ -73,7 +85,7 @@ __host__ __device__ inline PipeMeta* pl_meta(void* ws) { return (PipeMeta*)((uin
 
 struct PipeArgs {
     uint8_t* ws;
-    const uint32_t* map; // global tile -> column
+    const PlSlot* map; // block -> (column, tile)
     const PipeJob* jobs;
     WsHeader* hdr;
     PipeMeta* meta;
@@ -363,13 +375,13 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
     __shared__ uint32_t s_epoch, s_abort;
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t g = blockIdx.x; // global tile (one per block, in order)
-    // ---- the launch must match the prepared batch (the tile map lies
+    const uint32_t g = blockIdx.x;
+    // ---- the launch must match the prepared batch (the block map lies
     // within the workspace whatever it holds: read with the meta at once)
     const PipeMeta m = ldk(a.meta);
-    const uint32_t ji = ldk(a.map + g);
+    const PlSlot e = ldk(a.map + g);
     if (m.magic != PL_MAGIC) return; // never prepared: nothing to run or report
-    if (m.tiles != a.total_tiles || m.need > a.ws_bytes || ji >= m.count) {
+    if (m.grid != a.total_tiles || m.need > a.ws_bytes || (e.job != PL_IDLE && e.job >= m.count)) {
         // (the table as prepared -- its place depends on the prepared tile
         // count -- within the extent the prepare checked)
         const PipeJob* jobs = (const PipeJob*)(a.ws + m.off_jobs);
@@ -378,12 +390,13 @@ __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
                 if (jobs[c].stats) jobs[c].stats[0] = STATS_INVALID;
         return;
     }
+    if (e.job == PL_IDLE) return; // (an XCD's list shorter than the longest)
     if (tid == 0) {
         s_epoch = ld32(&a.hdr->epoch);
         s_abort = ld32(&a.hdr->abort);
     }
-    const PipeJob J = ldk(a.jobs + ji); // (uniform: scalar registers for the whole tile)
-    const uint32_t t = g - J.tile0;
+    const PipeJob J = ldk(a.jobs + e.job); // (uniform: scalar registers for the whole tile)
+    const uint32_t t = e.t;
     __syncthreads();
     if (s_abort) {
         if (t == J.T - 1 && tid == 0 && J.stats) gmem(J.stats)[0] = STATS_INVALID;
@@ -422,7 +435,9 @@ static uint64_t pl_lb_bytes(uint64_t tiles, uint64_t groups)
 }
 
 static uint64_t pl_jobs_bytes(uint32_t count) { return ((uint64_t)count * sizeof(PipeJob) + 255) / 256 * 256; }
-static uint64_t pl_map_bytes(uint64_t tiles) { return (tiles * 4 + 255) / 256 * 256; }
+static uint64_t pl_map_bytes(uint64_t grid) { return (grid * sizeof(PlSlot) + 255) / 256 * 256; }
+// the XCD-local layout is used only when its grid is at most this
+static uint64_t pl_grid_cap(uint64_t tiles) { return tiles + tiles / 10 + PL_XCDS; }
 constexpr uint64_t PL_OFF_MAP = 256 + WS_SMALL_BYTES;
 
 extern "C" uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t count)
@@ -434,7 +449,7 @@ extern "C" uint64_t am_pipeline_workspace_bytes(const uint64_t* n, uint32_t coun
         tiles += t;
         groups += t / 64 + 1;
     }
-    return PL_OFF_MAP + pl_map_bytes(tiles) + pl_jobs_bytes(count) + pl_lb_bytes(tiles, groups);
+    return PL_OFF_MAP + pl_map_bytes(pl_grid_cap(tiles)) + pl_jobs_bytes(count) + pl_lb_bytes(tiles, groups);
 }
 
 // Host-side validation + job table upload; after this, am_pipeline_run
@@ -463,21 +478,25 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
             if (v == 0) v = 32;
         }
         if (v != 8 && v != 16 && v != 32) return AM_E_INVALID;
-        // staged values: the output width rounded up to 1, 2 or 4 bytes; the
-        // tile grows by 4/vsz (s_vals holds PL_TMAX u32 = 4*PL_TMAX bytes)
+        // staged values: the output width rounded up to 1, 2 or 4 bytes, so
+        // a tile holds up to cap = PL_STAGE_BYTES / vsz values; variant 32
+        // takes the largest tile that fits, 16 and 8 that many elements per
+        // lane times 4 / vsz
         J.vsz = c.s.bits <= 8 ? 1 : c.s.bits <= 16 ? 2 : 4;
-        const uint32_t f = 4 / J.vsz, vf = v * f;
-        static_assert(PL_JMAX == 32, "the accepted tile sizes fit s_vals");
+        const uint32_t f = 4 / J.vsz, cap = PL_STAGE_BYTES / J.vsz;
+        const uint32_t jmax = cap / pl_tile(PL_SUB) * PL_SUB; // element path: whole sub-passes
+        const uint32_t vf = v == 32 ? jmax : v * f;
         J.J = vf;
         J.pr = make_pred(c.pred, J.a.mask);
         J.stats = c.stats;
         J.vbits = c.a.bits + 1;
         J.chk_ovf = c.s.bits < J.vbits ? 1 : 0;
-        // word path: whole slab words per tile, about J*256 elements: at
-        // least 2048 (the tile count am_pipeline_workspace_bytes allows for)
-        // and at most PL_TMAX (s_vals)
+        // word path: whole slab words per tile (256 per sub-pass), about as
+        // many elements as the element path's tile: at least 2048 (the tile
+        // count am_pipeline_workspace_bytes allows for), at most cap
+        const uint32_t spe = 256 * J.a.eps; // elements per sub-pass
         J.wsub = (J.a.kind == K_SLAB && c.a.bits >= 2)
-                     ? std::min(std::max(vf / J.a.eps, (8 + J.a.eps - 1) / J.a.eps), PL_JMAX * f / J.a.eps)
+                     ? std::min(std::max((v == 32 ? cap : pl_tile(vf)) / spe, (2048 + spe - 1) / spe), cap / spe)
                      : 0;
         J.tp = J.wsub ? make_top_pred(J.pr, c.a.bits, J.a.eps) : TopPred{0, 0, 0};
         if (J.wsub) {
@@ -494,20 +513,53 @@ extern "C" int am_pipeline_prepare(const am_pipeline_col* cols, uint32_t count, 
         groups += J.T / 64 + 1;
         if (tiles >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
     }
-    const uint64_t off_j = PL_OFF_MAP + pl_map_bytes(tiles), off_t = off_j + pl_jobs_bytes(count);
+    // Block map. XCD-local when it balances: columns go whole to the XCD
+    // with the fewest tiles so far (largest first), block i*8+x runs the
+    // i-th tile of XCD x's list, so a column's look-back traffic stays in
+    // one XCD's L2 and a tile still waits only on lower blocks of its own
+    // XCD. Otherwise (a few columns hold most tiles) block g runs global
+    // tile g, columns in order.
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return jobs[x].T > jobs[y].T; });
+    std::vector<std::vector<uint32_t>> lists(PL_XCDS);
+    std::vector<uint64_t> load(PL_XCDS, 0);
+    for (uint32_t i : order) {
+        const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[x].push_back(i);
+        load[x] += jobs[i].T;
+    }
+    const uint64_t maxl = *std::max_element(load.begin(), load.end());
+    const bool xcd = PL_XCDS * maxl <= pl_grid_cap(tiles);
+    const uint64_t grid = xcd ? PL_XCDS * maxl : tiles;
+    if (grid >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
+    std::vector<PlSlot> map(grid, PlSlot{PL_IDLE, 0});
+    if (xcd) {
+        for (uint32_t x = 0; x < PL_XCDS; x++) {
+            std::sort(lists[x].begin(), lists[x].end());
+            uint64_t i = 0;
+            for (uint32_t c : lists[x])
+                for (uint32_t t = 0; t < jobs[c].T; t++, i++) map[i * PL_XCDS + x] = PlSlot{c, t};
+        }
+    }
+    else {
+        for (uint32_t c = 0; c < count; c++)
+            for (uint32_t t = 0; t < jobs[c].T; t++) map[jobs[c].tile0 + t] = PlSlot{c, t};
+    }
+    const uint64_t off_j = PL_OFF_MAP + pl_map_bytes(grid), off_t = off_j + pl_jobs_bytes(count);
     const uint64_t need = off_t + pl_lb_bytes(tiles, groups);
     if (ws_bytes < need) return AM_E_WORKSPACE;
-    std::vector<uint32_t> map(tiles);
-    for (uint32_t i = 0; i < count; i++) std::fill(map.begin() + jobs[i].tile0, map.begin() + jobs[i].tile0 + jobs[i].T, i);
     // the previous run on this stream may still read the job table
     if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return AM_E_HIP;
     uint8_t* w8 = (uint8_t*)ws;
-    if (hipMemcpy(w8 + PL_OFF_MAP, map.data(), tiles * 4, hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
+    if (hipMemcpy(w8 + PL_OFF_MAP, map.data(), grid * sizeof(PlSlot), hipMemcpyHostToDevice) != hipSuccess)
+        return AM_E_HIP;
     if (hipMemcpy(w8 + off_j, jobs.data(), count * sizeof(PipeJob), hipMemcpyHostToDevice) != hipSuccess)
         return AM_E_HIP;
-    PipeMeta m{PL_MAGIC, count, (uint32_t)tiles, 0, need, off_j, off_t, off_t + tiles * PlLookback::TS * 8};
+    PipeMeta m{PL_MAGIC, count, (uint32_t)tiles, 0,   (uint32_t)grid, xcd ? 1u : 0u,
+               need,     off_j, off_t,               off_t + tiles * PlLookback::TS * 8};
     if (hipMemcpy(pl_meta(ws), &m, sizeof(m), hipMemcpyHostToDevice) != hipSuccess) return AM_E_HIP;
-    if (total_tiles) *total_tiles = (uint32_t)tiles;
+    if (total_tiles) *total_tiles = (uint32_t)grid;
     return AM_OK;
 }
 
@@ -525,7 +577,7 @@ extern "C" int am_pipeline_run(void* ws, uint64_t ws_bytes, uint32_t total_tiles
     a.ws = (uint8_t*)ws;
     a.hdr = (WsHeader*)ws;
     a.meta = pl_meta(ws);
-    a.map = (const uint32_t*)((uint8_t*)ws + PL_OFF_MAP);
+    a.map = (const PlSlot*)((uint8_t*)ws + PL_OFF_MAP);
     a.jobs = (const PipeJob*)((uint8_t*)ws + PL_OFF_MAP + pl_map_bytes(total_tiles));
     a.total_tiles = total_tiles;
     a.ws_bytes = ws_bytes;

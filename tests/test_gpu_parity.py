@@ -880,6 +880,47 @@ def test_pipeline_slab_every_width_and_predicate(am, oracle_mod):
     _check_pipeline(oracle_mod, refs, outs, st, "slab widths")
 
 
+def _pipeline_xcd_local(batch):
+    """The batch record's layout flag: word 3 of the workspace holds the
+    run's block count (low half) and whether each column runs on one XCD."""
+    return int(batch.ws[3].item()) >> 32
+
+
+def test_pipeline_xcd_local_and_fallback_layouts(am, oracle_mod):
+    """Many columns of similar length take the XCD-local block map (columns
+    whole on one XCD, padding blocks idle); a batch whose one column holds
+    most tiles keeps the column-by-column map. Both against the oracle,
+    run twice each on one workspace."""
+    import torch
+    rng = np.random.default_rng(515)
+    ops = {0: np.less, 2: np.greater}
+    for ncols, big in ((48, False), (6, True)):
+        cols, refs, variants = [], [], []
+        for i in range(ncols):
+            n = int(rng.integers(30000, 60000)) if not (big and i == 0) else 900001
+            w = int(rng.integers(2, 32))
+            a = rand_vals(rng, n, w)
+            b = rand_vals(rng, n, w)
+            op = (0, 2)[i % 2]
+            c = int(rng.integers(0, 1 << w))
+            fin = (1, 1, 2, 4)[i % 4]
+            fmt = (1, 2)[i % 2]
+            sb = w + 1 if i % 5 else max(1, w - 1)
+            cols.append((upload(am, oracle_mod, a, w, fin), upload(am, oracle_mod, b, w, fin), (op, c), fmt, sb))
+            v = (a + b) & np.uint64((1 << (w + 1)) - 1)
+            refs.append((v, ops[op](a, np.uint64(c)), fmt, sb))
+            variants.append((8, 16, 32)[i % 3])
+        say("pipeline layout", ncols, big)
+        batch = am.PipelineBatch(cols, variant=variants)
+        assert _pipeline_xcd_local(batch) == (0 if big else 1), (ncols, big)
+        for _ in range(2):
+            batch.stats.zero_()
+            batch.run()
+            torch.cuda.synchronize()
+            outs, st = batch.results()
+            _check_pipeline(oracle_mod, refs, outs, st, ("layout", ncols, big))
+
+
 def test_pipeline_graph_replay(am, oracle_mod):
     """Config C4's launch form: the batch captured once in a graph and
     replayed; before every replay the outputs and statistics are cleared
@@ -1001,7 +1042,7 @@ def test_pipeline_run_rejects_mismatched_launch(am, oracle_mod):
     rng = np.random.default_rng(92)
     cols, refs = _pipeline_cols(am, oracle_mod, rng, 5, 1)
     batch = am.PipelineBatch(cols, variant=16)
-    map_end = 256 + 65536 + (4 * batch.total_tiles + 255) // 256 * 256  # header, small region, tile map
+    map_end = 256 + 65536 + (8 * batch.total_tiles + 255) // 256 * 256  # header, small region, block map
     assert am.lib().am_pipeline_run(am.ctypes.c_void_p(batch.ws.data_ptr()), map_end - 8, batch.total_tiles,
                                     am._stream()) == 4  # AM_E_WORKSPACE: the map itself does not fit
     for tiles, wsb in ((batch.total_tiles + 1, batch.ws_bytes), (max(1, batch.total_tiles - 1), batch.ws_bytes),
