@@ -31,6 +31,7 @@
 // its arguments and does nothing but mark every column's statistics invalid
 // when they differ.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "am_device.hpp"
@@ -273,14 +274,21 @@ __device__ inline TileTot pl_select_words(const PipeJob& J, uint32_t t, V* s_val
                 pos += v < wave ? wc[v] : 0;
                 tot += wc[v];
             }
-            for (uint32_t f = 0, sh = 0; f < eps; f++, sh += w) {
-                const uint32_t v = ((uint32_t)(x >> sh) & m) + ((uint32_t)(y >> sh) & m); // < 2^32
-                const bool sel = (gg >> f) & 1;
-                if (sel) s_vals[pos] = (V)(v & om);
-                pos += sel;
-                vsum += sel ? v : 0;
-                ovf += (sel && (v & ~om) != 0) ? 1 : 0; // (0 when s holds w+1 bits)
-            }
+            // (an output at least w+1 bits wide holds every a+b: no masking,
+            // no overflow count -- a separate instance of the field loop)
+            auto fields = [&](auto chkc) {
+                constexpr bool CHK = decltype(chkc)::value;
+                for (uint32_t f = 0, sh = 0; f < eps; f++, sh += w) {
+                    const uint32_t v = ((uint32_t)(x >> sh) & m) + ((uint32_t)(y >> sh) & m); // < 2^32
+                    const bool sel = __builtin_amdgcn_ubfe(gg, f, 1) != 0;
+                    if (sel) s_vals[pos] = (V)(CHK ? (v & om) : v);
+                    pos += sel ? 1u : 0u;
+                    vsum += sel ? v : 0u;
+                    if (CHK) ovf += (sel && (v & ~om) != 0) ? 1 : 0;
+                }
+            };
+            if (J.chk_ovf) fields(std::true_type{});
+            else fields(std::false_type{});
             base += tot;
         }
     }
