@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "../../include/amethyst.h"
 
 namespace am {
@@ -281,6 +283,25 @@ __device__ inline uint32_t mbcnt(uint64_t m)
 }
 
 __device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+// f(integral_constant<J>) for J in the sequence, unrolled at compile time
+template <uint32_t... J, typename Fn>
+__device__ inline void static_for(std::integer_sequence<uint32_t, J...>, Fn&& f)
+{
+    (f(std::integral_constant<uint32_t, J>{}), ...);
+}
+
+// Lane J of (lo, hi) := the uniform 64-bit value v (two v_writelane_b32 with
+// an immediate lane select). gfx950 needs 2 wait states between a VALU write
+// of an SGPR (such as the compare that produced v) and a VALU read of it; the
+// compiler cannot see inside the asm, so the s_nop 1 is explicit.
+template <uint32_t J>
+__device__ inline void writelane_imm64(uint32_t& lo, uint32_t& hi, uint64_t v)
+{
+    asm volatile("s_nop 1\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
+                 : "+v"(lo), "+v"(hi)
+                 : "s"((uint32_t)v), "s"((uint32_t)(v >> 32)), "i"(J));
+}
 
 __device__ inline uint64_t shfl_xor_u64(uint64_t v, int m)
 {

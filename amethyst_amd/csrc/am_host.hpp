@@ -109,6 +109,25 @@ inline TopPred make_top_pred(const Pred& p, uint32_t bits, uint32_t eps)
     return t;
 }
 
+// The selected values of a predicate over `bits` <= 32-bit values as one
+// cyclic range [c0, c0 + m) (mod 2^bits), scaled to values top-aligned in 32
+// bits with arbitrary bits beneath: selected <=> (y - lo) mod 2^32 <= rg.
+// One range covers every operator, NE included (its complement of one value
+// wraps around). Returns false when nothing is selected.
+inline bool cyclic_top_range(const Pred& p, uint32_t bits, uint32_t* lo, uint32_t* rg)
+{
+    const uint64_t vmax = lowmask(bits);
+    uint64_t c0 = p.lo, m = p.span >= vmax ? vmax + 1 : p.span + 1;
+    if (p.inv) {
+        c0 = (p.lo + m) & vmax;
+        m = vmax + 1 - m;
+    }
+    const uint32_t k = 32 - bits;
+    *lo = (uint32_t)(c0 << k);
+    *rg = (uint32_t)((m << k) - 1);
+    return m != 0;
+}
+
 inline int check_launch()
 {
     hipError_t e = hipGetLastError();

@@ -73,17 +73,33 @@ __global__ __launch_bounds__(256) void k_filter(Col in, Pred pr, uint64_t* mask_
 
 
 // Bitstream inputs of <= 32 bits: a block owns 4096 elements; it copies their
-// input words into LDS with coalesced loads, then lane l of a wave takes
-// element 64j + l with one funnel shift (v_alignbit) of two LDS dwords, and
-// the wave's ballot of the predicate is mask word j. One tile per block.
+// input words into LDS with coalesced loads (one pad word in front), then
+// lane l of a wave takes element 64j + l as the 32 stream bits ending at the
+// element's last bit -- one funnel shift (v_alignbit) of two LDS dwords puts
+// the field at the top with the previous element's bits beneath, which a
+// range test scaled by 2^(32-bits) ignores -- and the wave's ballot of the
+// test is mask word j. Chunk j+1's element sits 64*bits bits later, a whole
+// number of dwords, so the shift is the lane's for every chunk and only the
+// LDS address moves; the range is cyclic (one test for every operator, NE
+// included). One tile per block.
 constexpr uint32_t FS_J = 16;                   // mask words per wave
 constexpr uint32_t FS_TILE = 4 * FS_J * 64;     // elements per block
 constexpr uint32_t FS_IN = FS_TILE * 32 / 64 + 3; // staged input words (bits <= 32)
 
-__global__ __launch_bounds__(256) void k_filter_stream(Col in, Pred pr, uint64_t* mask_out)
+struct FilterStreamArgs {
+    Col in;
+    uint32_t clo, crg; // selected <=> (y - clo) mod 2^32 <= crg, y the field top-aligned
+    uint32_t none;     // the predicate selects nothing
+    uint64_t* mask_out;
+};
+
+template <bool LO0>
+__global__ __launch_bounds__(256) void k_filter_stream(FilterStreamArgs a)
 {
-    __shared__ uint64_t s_w[FS_IN];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ uint64_t s_w[FS_IN + 1]; // s_w[0]: pad below the first element
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const Col& in = a.in;
     const uint64_t n = in.n, t0 = (uint64_t)blockIdx.x * FS_TILE;
     const uint32_t w = in.bits;
     const uint64_t ib0 = (t0 * w) >> 6;
@@ -99,30 +115,45 @@ __global__ __launch_bounds__(256) void k_filter_stream(Col in, Pred pr, uint64_t
 #pragma unroll
         for (uint32_t k = 0; k < PER; k++) {
             const uint32_t i = tid + 256 * k;
-            v[k] = i < nin ? in.p[ib0 + i] : 0;
+            v[k] = i < nin ? gmem(in.p)[ib0 + i] : 0;
         }
 #pragma unroll
         for (uint32_t k = 0; k < PER; k++) {
             const uint32_t i = tid + 256 * k;
-            if (i < FS_IN) s_w[i] = v[k];
+            if (i < FS_IN) s_w[1 + i] = v[k];
         }
+        if (tid == 0) s_w[0] = 0;
     }
     __syncthreads();
     const uint32_t* s32 = (const uint32_t*)s_w;
-    const uint32_t rb = (uint32_t)(t0 * w - ib0 * 64); // input bit of element t0 in the staged run
+    const uint32_t rb = (uint32_t)(t0 * w - ib0 * 64); // stream bit of element t0 in the staged run
     const uint32_t ne = (uint32_t)(tend - t0);
-    const uint32_t vm = (uint32_t)in.mask;
-    uint64_t mine = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < FS_J; j++) {
-        const uint32_t er = wave * (FS_J * 64) + j * 64 + lane;
-        const uint32_t b = rb + er * w; // < 2^18: its dword pair lies in s_w
-        const uint32_t v = __builtin_amdgcn_alignbit(s32[(b >> 5) + 1], s32[b >> 5], b & 31) & vm;
-        const uint64_t m = ballot(er < ne && pr.sel32(v));
-        if (lane == j) mine = m;
-    }
+    // end bit (exclusive) of the lane's element of chunk 0, relative to the run
+    const uint32_t e0 = rb + (wave * (FS_J * 64) + lane + 1) * w;
+    const uint32_t sh = e0 & 31;
+    const uint32_t* const p = s32 + (e0 >> 5) + 1; // the window's low dword (run dword 0 is s32[2])
+    const uint32_t CLO = a.clo, CRG = a.crg;
+    const bool last = blockIdx.x == gridDim.x - 1;
+    uint32_t mlo = 0, mhi = 0; // lane j: the wave's mask word j
+    // every chunk's window first (one LDS round trip for the wave's 16)
+    uint32_t y[FS_J];
+    static_for(std::make_integer_sequence<uint32_t, FS_J>{}, [&](auto jc) {
+        constexpr uint32_t j = decltype(jc)::value;
+        const uint32_t* q = p + 2 * w * j;
+        y[j] = __builtin_amdgcn_alignbit(q[1], q[0], sh);
+    });
+    static_for(std::make_integer_sequence<uint32_t, FS_J>{}, [&](auto jc) {
+        constexpr uint32_t j = decltype(jc)::value;
+        uint64_t b = ballot(LO0 ? (y[j] <= CRG) : (y[j] - CLO <= CRG));
+        if (last) { // elements past n (only in the last block)
+            const uint32_t e = wave * (FS_J * 64) + j * 64;
+            b &= e >= ne ? 0 : (ne - e >= 64 ? ~0ull : lowmask(ne - e));
+        }
+        if (a.none) b = 0;
+        writelane_imm64<j>(mlo, mhi, b);
+    });
     const uint64_t mw = (t0 >> 6) + wave * FS_J + lane;
-    if (lane < FS_J && mw < ((n + 63) >> 6)) st_nt(mask_out + mw, mine);
+    if (lane < FS_J && mw < ((n + 63) >> 6)) st_nt(a.mask_out + mw, ((uint64_t)mhi << 32) | mlo);
 }
 
 // Word-per-lane filter for slab and container inputs: a block owns a tile
@@ -517,7 +548,12 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
     if (c.kind == K_BITSTREAM && c.bits <= 32) {
         const uint64_t TS = (c.n + FS_TILE - 1) / FS_TILE;
         if (TS >= (1ull << 31)) return AM_E_UNSUPPORTED_CONFIG;
-        hipLaunchKernelGGL(k_filter_stream, dim3((uint32_t)TS), dim3(256), 0, s, c, pr, mask_out);
+        FilterStreamArgs fa{};
+        fa.in = c;
+        fa.none = cyclic_top_range(pr, c.bits, &fa.clo, &fa.crg) ? 0 : 1;
+        fa.mask_out = mask_out;
+        if (fa.clo == 0) hipLaunchKernelGGL(k_filter_stream<true>, dim3((uint32_t)TS), dim3(256), 0, s, fa);
+        else hipLaunchKernelGGL(k_filter_stream<false>, dim3((uint32_t)TS), dim3(256), 0, s, fa);
         return check_launch();
     }
 #define FT_GO(IK, W)                                                                                        \
