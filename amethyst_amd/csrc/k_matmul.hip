@@ -44,6 +44,7 @@ constexpr uint32_t MM_SLOT = 64 * MM_C_STRIDE * 4; // one stage; in the epilogue
 static_assert(MM_SLOT >= MM_STAGE, "a stage fits its slot");
 // s_waitcnt vmcnt(0) with the other counters unconstrained (vmcnt bits 3:0 and 15:14)
 constexpr unsigned MM_WAIT_VM0 = 0x0F70;
+constexpr unsigned MM_WAIT_VM0_LGKM0 = 0x0070; // vmcnt(0) lgkmcnt(0) (bits 11:8), expcnt unconstrained
 constexpr uint32_t MM_THREADS = 512; // 8 waves: 4 x 64 rows by 2 x 128 columns of the tile
 
 struct MatJob {
@@ -252,8 +253,9 @@ __device__ inline MatJob uniform_job(const MatJob& j)
 // K-steps of 128 bytes land in LDS straight from global memory (buffer loads
 // with the LDS flag, 16 bytes per lane, no registers), one step ahead, in two
 // stage arrays used alternately. A stage holds 256 rows of A then 256 of B,
-// 128 bytes each; chunk c (16 bytes) of row R sits in slot c ^ (R % 8) (an XOR
-// swizzle: the fragment reads of 8 consecutive rows hit 8 different slots).
+// 128 bytes each; chunk c (16 bytes) of row R sits in slot c ^ (R % 8) ^ (R / 8 % 2)
+// (an XOR swizzle: any 16 consecutive rows, 2 KB, put their chunk c in 16
+// distinct 16-byte bank groups, so the fragment reads meet no bank twice).
 // One load instruction fills 1 KB = 8 rows: lane l takes row l/8, slot l%8.
 // Wave w loads rows 32w..32w+31 of A and of B. Loads bounded by the tile's
 // valid rows read zeros past M (or N).
@@ -289,9 +291,9 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
         const uint32_t lrow = lane >> 3;
         const uint32_t vo = (wave * 32 + lrow) * J.Kp + ((lane & 7) ^ lrow) * 16, rs = 8 * J.Kp;
         auto gload = [&](uint32_t ks, int8_t* st) {
-            const uint32_t ko = vo + ks * MM_BK;
 #pragma unroll
             for (uint32_t h = 0; h < 4; h++) {
+                const uint32_t ko = (vo ^ ((h & 1) * 16)) + ks * MM_BK; // rows 8h..: slot ^ 1 when h is odd
                 const uint32_t lo = (wave * 4 + h) * 1024;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, (__attribute__((address_space(3))) void*)(st + lo), 16,
                                                          ko + h * rs, 0, 0, 0);
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
         };
         // fragment reads: row R = 128*wm (or wn) + 32*i + l32, k chunk 2*sub + lh
         // (k = 32*sub .. 32*sub+31 of the step; both operands map k to lanes alike)
-        const uint32_t fbase = l32 * MM_BK, fx = l32 & 7;
+        const uint32_t fbase = l32 * MM_BK, fx = (l32 ^ ((l32 >> 3) & 1)) & 7;
 
         v16i acc[2][4];
 #pragma unroll
@@ -309,34 +311,60 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
 #pragma unroll
             for (uint32_t j = 0; j < 4; j++) acc[i][j] = v16i{};
 
-        // one K-step: refill the other stage with step ks+1, compute from
-        // `cur`, then wait for this wave's loads and meet the other waves
-        auto step = [&](uint32_t ks, const int8_t* cur, int8_t* nxt) {
-            if (ks + 1 < steps) gload(ks + 1, nxt);
+        // two sets of fragment registers: the reads of sub-step s+1 go out
+        // between the matrix ops of sub-step s, so the matrix core is not left
+        // waiting for LDS at the start of each sub-step
+        v4i fa[2][2], fb[2][4];
+        auto fread = [&](const int8_t* st, uint32_t sub, uint32_t b) {
+            const uint32_t fo = fbase + (((2 * sub + lh) ^ fx) * 16);
 #pragma unroll
-            for (uint32_t sub = 0; sub < 4; sub++) {
-                const uint32_t fo = fbase + (((2 * sub + lh) ^ fx) * 16);
-                v4i fa[2], fb[4];
+            for (uint32_t i = 0; i < 2; i++) fa[b][i] = *(const v4i*)(st + (wm * 64 + i * 32) * MM_BK + fo);
 #pragma unroll
-                for (uint32_t i = 0; i < 2; i++) fa[i] = *(const v4i*)(cur + (wm * 64 + i * 32) * MM_BK + fo);
+            for (uint32_t j = 0; j < 4; j++) fb[b][j] = *(const v4i*)(st + MM_BM * MM_BK + (wn * 128 + j * 32) * MM_BK + fo);
+        };
+        auto mfmas = [&](uint32_t b) {
+#pragma unroll
+            for (uint32_t i = 0; i < 2; i++)
 #pragma unroll
                 for (uint32_t j = 0; j < 4; j++)
-                    fb[j] = *(const v4i*)(cur + MM_BM * MM_BK + (wn * 128 + j * 32) * MM_BK + fo);
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[b][i], fb[b][j], acc[i][j], 0, 0, 0);
+        };
+        // schedule of one sub-step: matrix op, LDS read, ... (6 reads), then
+        // the last two matrix ops; the region ends at the sched_barrier
+        auto interleave = [&]() {
 #pragma unroll
-                for (uint32_t i = 0; i < 2; i++)
-#pragma unroll
-                    for (uint32_t j = 0; j < 4; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-                __builtin_amdgcn_sched_barrier(0); // keep each sub-step's fragments to itself (register pressure)
+            for (int k = 0; k < 6; k++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0); // one LDS read
             }
-            // step ks+1 has landed for this wave; after the barrier for all
-            // waves, and every wave is done reading `cur`. A bare s_barrier
-            // (no fences; the wait above is the one needed); the empty asm
-            // keeps the compiler from moving LDS accesses across it.
-            __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        };
+
+        // one K-step from `cur` (sub-step 0's fragments already read); the
+        // other stage `nxt` holds or receives step ks+1, loaded since the
+        // previous step's barrier. Sub-steps 0..2 read the next sub-step's
+        // fragments; then the wave waits for its loads of step ks+1 and its
+        // reads of `cur`, meets the other waves (after which every wave is
+        // done with `cur` and step ks+1 is in LDS), refills `cur` with step
+        // ks+2 and reads step ks+1's first fragments during sub-step 3. A bare
+        // s_barrier (no fences; the waits before it are the ones needed); the
+        // empty asm keeps the compiler from moving LDS accesses across it.
+        auto step = [&](uint32_t ks, int8_t* cur, const int8_t* nxt) {
+#pragma unroll
+            for (uint32_t sub = 0; sub < 3; sub++) {
+                fread(cur, sub + 1, (sub + 1) & 1);
+                mfmas(sub & 1);
+                interleave();
+            }
+            __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0_LGKM0);
             asm volatile("" ::: "memory");
             __builtin_amdgcn_s_barrier();
             asm volatile("" ::: "memory");
+            if (ks + 2 < steps) gload(ks + 2, cur);
+            fread(nxt, 0, 0); // (after the last step: stale data, never used)
+            mfmas(1);
+            interleave();
         };
 
         __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0); // the previous tile's stores are out of the count
@@ -344,10 +372,14 @@ __global__ __launch_bounds__(MM_THREADS, 1) void k_mm_gemm(const MatJob* jobs, u
         gload(0, st0);
         __builtin_amdgcn_s_waitcnt(MM_WAIT_VM0);
         __syncthreads();
+        if (1 < steps) gload(1, st1);
+        fread(st0, 0, 0);
         for (uint32_t ks = 0; ks < steps; ks += 2) {
             step(ks, st0, st1);
             if (ks + 1 < steps) step(ks + 1, st1, st0);
         }
+        // (the fragment reads after the last barrier fetch data nobody uses;
+        // the epilogue's LDS writes may overtake them)
 
         // ---- epilogue through LDS, one 128-row half of the tile at a time:
         // the four waves owning the half write their accumulators (lane holds
