@@ -1,6 +1,6 @@
 // k_pipeline.hip -- the per-column operator chain
 //     mask = filter(a, pred); s = compressstore(a, mask) + compressstore(b, mask);
-//     total = sum(s)
+//     total = sum of the selected a+b (exact, whatever width s is stored at)
 // for a whole batch of columns of uneven length, width and tile size in ONE
 // launch, capturable in a hipGraph.
 //
@@ -237,6 +237,19 @@ __device__ inline TileTot pl_select_words(const PipeJob& J, uint32_t t, V* s_val
     const AM_GLOBAL uint64_t* pb = gmem(J.b.p);
     // the tile reaches past the column's last element: mask fields >= n
     const bool tail = (tw0 + (uint64_t)nsub * 256) * eps > n;
+    // SWAR sums: me covers the even fields 0, 2, 4, ... of a word, mo the
+    // even fields f that have an odd field f+1 above them; a sum of two
+    // w-bit fields takes w+1 bits, its carry landing in the (masked-out)
+    // place of the field above
+    uint64_t me = 0, mo = 0;
+    for (uint32_t f = 0; f < eps; f += 2) {
+        me |= (uint64_t)m << (f * w);
+        if (f + 1 < eps) mo |= (uint64_t)m << (f * w);
+    }
+    const uint32_t m1 = w >= 31 ? 0xFFFFFFFFu : (2u << w) - 1; // w+1 bits
+    // unselected fields are stored too, each lane to a slot of its own past
+    // the staging area (no branch around the store)
+    const uint32_t trash = PL_STAGE_BYTES / (uint32_t)sizeof(V) + lane * (4 / (uint32_t)sizeof(V));
     uint64_t X[PL_RING], Y[PL_RING];
     auto fetch = [&](uint32_t k, uint32_t sp) {
         const uint64_t wi = tw0 + sp * 256 + tid;
@@ -274,17 +287,25 @@ __device__ inline TileTot pl_select_words(const PipeJob& J, uint32_t t, V* s_val
                 pos += v < wave ? wc[v] : 0;
                 tot += wc[v];
             }
+            // a+b of every field of the word: even fields in E, odd fields
+            // (shifted down by w) in O, each at its even field's place
+            const uint64_t E = (x & me) + (y & me);
+            const uint64_t O = ((x >> w) & mo) + ((y >> w) & mo);
             // (an output at least w+1 bits wide holds every a+b: no masking,
             // no overflow count -- a separate instance of the field loop)
             auto fields = [&](auto chkc) {
                 constexpr bool CHK = decltype(chkc)::value;
-                for (uint32_t f = 0, sh = 0; f < eps; f++, sh += w) {
-                    const uint32_t v = ((uint32_t)(x >> sh) & m) + ((uint32_t)(y >> sh) & m); // < 2^32
+                auto one = [&](uint64_t S, uint32_t f, uint32_t sh) {
+                    const uint32_t v = (uint32_t)(S >> sh) & m1; // < 2^32
                     const bool sel = __builtin_amdgcn_ubfe(gg, f, 1) != 0;
-                    if (sel) s_vals[pos] = (V)(CHK ? (v & om) : v);
+                    s_vals[sel ? pos : trash] = (V)(CHK ? (v & om) : v);
                     pos += sel ? 1u : 0u;
                     vsum += sel ? v : 0u;
                     if (CHK) ovf += (sel && (v & ~om) != 0) ? 1 : 0;
+                };
+                for (uint32_t f = 0, sh = 0; f < eps; f += 2, sh += 2 * w) {
+                    one(E, f, sh);
+                    if (f + 1 < eps) one(O, f + 1, sh);
                 }
             };
             if (J.chk_ovf) fields(std::true_type{});
@@ -376,7 +397,9 @@ __device__ inline void pl_finish(const PipeArgs& a, const PipeMeta& m, const Pip
 
 __global__ __launch_bounds__(256) void k_pipeline(PipeArgs a)
 {
-    __shared__ uint32_t s_vals[PL_TMAX]; // a+b of <= 31-bit inputs fits 32 bits
+    // a+b of <= 31-bit inputs fits 32 bits; 64 more words take the word
+    // path's stores of unselected fields
+    __shared__ uint32_t s_vals[PL_TMAX + 64];
     __shared__ uint32_t s_wcnt[2 * PL_WAVES], s_wovf[PL_WAVES];
     __shared__ uint64_t s_wsum[PL_WAVES];
     __shared__ uint64_t s_P, s_carry;
