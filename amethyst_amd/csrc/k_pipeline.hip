@@ -216,12 +216,12 @@ __device__ inline TileTot pl_select_elems(const PipeJob& J, uint32_t t, V* s_val
 // Phase 1 of the word path (slab a and b of 2..31 bits, eps = 64/bits
 // fields per word): the tile is J.wsub sub-passes of 256 words, lane
 // `tid` takes word tid of each (so a wave's words, and their rows, are
-// consecutive); the words of the next four sub-passes are in flight while
-// one is used. Per word: the selection bits of its eps fields at once
+// consecutive); the words of the next eight sub-passes are in flight while
+// one is used (four: 2 % slower on the 512-column batch). Per word: the selection bits of its eps fields at once
 // (select_fields: one compare and one add-with-carry per field), the rank
 // of the lane's first selected row by a DPP scan of the per-lane counts
-// plus the waves before, then a+b of each selected field to s_vals.
-constexpr uint32_t PL_RING = 4;
+// plus the waves before, then a+b of every field (SWAR) to s_vals.
+constexpr uint32_t PL_RING = 8;
 template <typename V>
 __device__ inline TileTot pl_select_words(const PipeJob& J, uint32_t t, V* s_vals, uint32_t* s_wcnt,
                                           uint32_t* s_wovf, uint64_t* s_wsum)
