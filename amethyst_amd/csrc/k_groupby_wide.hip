@@ -37,6 +37,7 @@ constexpr uint32_t GW_MAXLB = 12;        // low key bits per LDS table
 constexpr uint32_t GW_LC = 15;           // log2 of the pairs per aggregation chunk
 constexpr uint32_t GW_C = 1u << GW_LC;  // (a count fits 16 bits, a sum vbits + 15)
 constexpr uint32_t GW_MAXG = 1024;       // row blocks of the partition passes
+constexpr uint32_t GW_AGG_U = 8;         // pairs per thread in flight in the aggregation
 
 struct WideArgs {
     Col keys, vals;
@@ -109,7 +110,23 @@ __device__ inline uint64_t block_excl_u64(uint64_t v, uint64_t* s_w, uint64_t* t
 
 // Rows [b*rpb, ...) of the block, GW_ROWS per thread per step; fn(key, value)
 // for each row that passes the value predicate. Values are read only when
-// the predicate or the caller needs them.
+// the predicate or the caller needs them. The next step's rows are loaded
+// before this step's are used (a block's steps would otherwise each wait a
+// full memory round trip).
+template <int KK, int VK>
+__device__ inline void gw_load(const WideArgs& a, uint64_t t0, uint64_t r1, bool wantv, uint64_t* kv, uint64_t* vv)
+{
+    const TileReader<KK> rk(a.keys, t0);
+    const TileReader<VK> rv(a.vals, t0);
+    const uint64_t rem = r1 - t0;
+#pragma unroll
+    for (uint32_t r = 0; r < GW_ROWS; r++) {
+        const uint32_t i = r * GW_TH + threadIdx.x;
+        kv[r] = i < rem ? rk.get(i) : 0;
+        vv[r] = (wantv && i < rem) ? rv.get(i) : 0;
+    }
+}
+
 template <int KK, int VK, bool NEEDV, typename Fn>
 __device__ inline void gw_rows(const WideArgs& a, Fn fn)
 {
@@ -118,21 +135,20 @@ __device__ inline void gw_rows(const WideArgs& a, Fn fn)
     const uint64_t r0 = (uint64_t)blockIdx.x * a.rpb;
     const uint64_t r1 = r0 + a.rpb < n ? r0 + a.rpb : n;
     const bool wantv = NEEDV || a.use_pred;
+    uint64_t kv[GW_ROWS], vv[GW_ROWS], kn[GW_ROWS], vn[GW_ROWS];
+    if (r0 < r1) gw_load<KK, VK>(a, r0, r1, wantv, kv, vv);
     for (uint64_t t0 = r0; t0 < r1; t0 += TILE) {
-        const TileReader<KK> rk(a.keys, t0);
-        const TileReader<VK> rv(a.vals, t0);
+        if (t0 + TILE < r1) gw_load<KK, VK>(a, t0 + TILE, r1, wantv, kn, vn);
         const uint64_t rem = r1 - t0;
-        uint64_t kv[GW_ROWS], vv[GW_ROWS];
-#pragma unroll
-        for (uint32_t r = 0; r < GW_ROWS; r++) {
-            const uint32_t i = r * GW_TH + threadIdx.x;
-            kv[r] = i < rem ? rk.get(i) : 0;
-            vv[r] = (wantv && i < rem) ? rv.get(i) : 0;
-        }
 #pragma unroll
         for (uint32_t r = 0; r < GW_ROWS; r++) {
             const uint32_t i = r * GW_TH + threadIdx.x;
             if (i < rem && (!a.use_pred || a.pr(vv[r]))) fn(kv[r], vv[r]);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < GW_ROWS; r++) {
+            kv[r] = kn[r];
+            vv[r] = vn[r];
         }
     }
 }
@@ -251,17 +267,18 @@ __device__ inline void gbw_scatter(const WideArgs& a)
     const uint64_t n = a.keys.n;
     const uint64_t r0 = (uint64_t)blockIdx.x * a.rpb;
     const uint64_t r1 = r0 + a.rpb < n ? r0 + a.rpb : n;
+    uint64_t kv[GW_ROWS], vv[GW_ROWS], kn[GW_ROWS], vn[GW_ROWS];
+    if (r0 < r1) gw_load<KK, VK>(a, r0, r1, true, kv, vv);
     for (uint64_t t0 = r0; t0 < r1; t0 += TILE) {
-        const TileReader<KK> rk(a.keys, t0);
-        const TileReader<VK> rv(a.vals, t0);
-        const uint64_t rem = r1 - t0;
-        uint64_t kv[GW_ROWS], vv[GW_ROWS];
+        if (t0 > r0) { // the rows loaded during the previous step
 #pragma unroll
-        for (uint32_t r = 0; r < GW_ROWS; r++) {
-            const uint32_t i = r * GW_TH + threadIdx.x;
-            kv[r] = i < rem ? rk.get(i) : 0;
-            vv[r] = i < rem ? rv.get(i) : 0;
+            for (uint32_t r = 0; r < GW_ROWS; r++) {
+                kv[r] = kn[r];
+                vv[r] = vn[r];
+            }
         }
+        if (t0 + TILE < r1) gw_load<KK, VK>(a, t0 + TILE, r1, true, kn, vn);
+        const uint64_t rem = r1 - t0;
         if (!SORT) {
 #pragma unroll
             for (uint32_t r = 0; r < GW_ROWS; r++) {
@@ -347,19 +364,39 @@ __global__ __launch_bounds__(GW_TH) void k_gbw_agg(WideArgs a)
     for (uint32_t k = threadIdx.x; k < K * copies; k += GW_TH) s_t[k] = 0;
     __syncthreads();
     const uint64_t vm = lowmask(vb);
+    // (GW_AGG_U pairs per thread loaded before any is added)
+    constexpr uint32_t U = GW_AGG_U;
     if (a.pair64) {
         const uint64_t* pr = (const uint64_t*)a.pairs + q0;
-        for (uint32_t j = threadIdx.x; j < cnt; j += GW_TH) {
-            const uint64_t x = pr[j];
-            atomicAdd((unsigned long long*)&s_t[cp * K + (uint32_t)(x >> vb)], (unsigned long long)(ONE | (x & vm)));
+        for (uint32_t j0 = 0; j0 < cnt; j0 += GW_TH * U) {
+            uint64_t x[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t j = j0 + u * GW_TH + threadIdx.x;
+                x[u] = j < cnt ? pr[j] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                if (j0 + u * GW_TH + threadIdx.x < cnt)
+                    atomicAdd((unsigned long long*)&s_t[cp * K + (uint32_t)(x[u] >> vb)],
+                              (unsigned long long)(ONE | (x[u] & vm)));
         }
     }
     else {
         const uint32_t* pr = (const uint32_t*)a.pairs + q0;
-        for (uint32_t j = threadIdx.x; j < cnt; j += GW_TH) {
-            const uint32_t x = pr[j];
-            const uint32_t k = vb < 32 ? (x >> vb) : 0;
-            atomicAdd((unsigned long long*)&s_t[cp * K + k], (unsigned long long)(ONE | (x & (uint32_t)vm)));
+        for (uint32_t j0 = 0; j0 < cnt; j0 += GW_TH * U) {
+            uint32_t x[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t j = j0 + u * GW_TH + threadIdx.x;
+                x[u] = j < cnt ? pr[j] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                if (j0 + u * GW_TH + threadIdx.x < cnt) {
+                    const uint32_t k = vb < 32 ? (x[u] >> vb) : 0;
+                    atomicAdd((unsigned long long*)&s_t[cp * K + k], (unsigned long long)(ONE | (x[u] & (uint32_t)vm)));
+                }
         }
     }
     __syncthreads();
