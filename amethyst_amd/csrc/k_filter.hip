@@ -493,6 +493,36 @@ __global__ __launch_bounds__(256) void k_sum_partial(Col in, uint64_t* partial)
     if (threadIdx.x == 0) partial[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 }
 
+// 8-byte containers (the baseline format, 64-bit strided columns) at a
+// 16-byte aligned base: each lane sums 16-byte pieces (two elements), eight
+// in flight, with non-temporal loads; an odd last element is added by
+// block 0's first thread
+constexpr uint32_t SUM_U64_UNROLL = 8;
+__global__ __launch_bounds__(256) void k_sum_u64(const uint64_t* p, uint64_t n, uint64_t vmask, uint64_t* partial)
+{
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+    __shared__ uint64_t s_part[4];
+    const u64x2* q = (const u64x2*)p;
+    const uint64_t pairs = n / 2;
+    const uint64_t gstride = (uint64_t)gridDim.x * 256 * SUM_U64_UNROLL;
+    uint64_t acc = 0;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * 256 * SUM_U64_UNROLL + threadIdx.x; i0 < pairs; i0 += gstride) {
+        u64x2 w[SUM_U64_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < SUM_U64_UNROLL; u++) {
+            const uint64_t i = i0 + u * 256;
+            w[u] = i < pairs ? __builtin_nontemporal_load(q + i) : u64x2{0, 0};
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < SUM_U64_UNROLL; u++) acc += (w[u].x & vmask) + (w[u].y & vmask);
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) acc += p[n - 1] & vmask;
+    acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+}
+
 __global__ __launch_bounds__(256) void k_sum_final(const uint64_t* partial, uint32_t nparts, uint64_t* out)
 {
     __shared__ uint64_t s_part[4];
@@ -641,6 +671,14 @@ extern "C" int am_sum(const am_column* in, uint64_t* sum_dev, void* ws, uint64_t
         if (sw.K < SUM_UNROLL) sw.K = SUM_UNROLL;
         hipLaunchKernelGGL(k_sum_bits_swar, g, b, 0, s, sw, c.n * c.bits, partial);
         hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partial, (uint32_t)blocks, sum_dev);
+        return check_launch();
+    }
+    if (c.kind == K_STRIDED && c.cbytes == 8 && ((uintptr_t)c.p & 15) == 0 && c.n) {
+        uint64_t ub = (c.n / 2 + 256 * SUM_U64_UNROLL - 1) / (256 * SUM_U64_UNROLL);
+        if (ub < 1) ub = 1;
+        if (ub > bcap) ub = bcap;
+        hipLaunchKernelGGL(k_sum_u64, dim3((uint32_t)ub), b, 0, s, (const uint64_t*)c.p, c.n, c.mask, partial);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partial, (uint32_t)ub, sum_dev);
         return check_launch();
     }
     switch (c.kind) {

@@ -36,8 +36,9 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--quick", action="store_true", help="headline only, no sweep")
-    p.add_argument("--legs", default="sweep,groupby,matmul,batched",
-                   help="comma-separated legs to run after the headline (sweep, groupby, matmul, batched)")
+    p.add_argument("--legs", default="sweep,groupby,matmul,batched,libbase",
+                   help="comma-separated legs to run after the headline (sweep, groupby, matmul, batched, "
+                        "libbase: hipCUB select/reduce over u64 elements beside the library's operators)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--reps", type=int, default=10, help="launches per sweep point")
     p.add_argument("--csv", default=None, metavar="DIR",
@@ -405,6 +406,65 @@ def matmul_leg(am, torch, timer, reps):
     return res
 
 
+def libbase_leg(am, torch, timer, reps):
+    """The reference's GPU library baselines, run on this GPU: hipCUB (rocPRIM)
+    over one u64 per element -- DeviceSelect::Flagged with the bitmask
+    (src/benchs/bench_compressstore_baseline_cub.cu:29-35) and DeviceReduce::Sum
+    (src/benchs/bench_hop_baseline_cub.cu:32-38) -- beside the library's
+    operators on the same 2^28 values (uniform 11-bit, the headline's v < 1024
+    selection): unpacked (u64 in and out) and packed (11-bit slabs)."""
+    from amethyst_amd import baseline as hb
+    n = N_FULL
+    col64 = am.gen_uniform(42, n, 11, am.BASELINE)
+    slab = am.gen_uniform(42, n, 11, am.SLAB)
+    mask = am.filter(slab, am.LT, 1024)
+    sel = hb.SelectFlagged(col64.buf, mask, n)
+    flags8 = (col64.buf < 1024).to(torch.uint8)  # the same selection, one byte per element
+    sel8 = hb.SelectFlagged(col64.buf, flags8, n, bytes=True)
+    red = hb.ReduceSum(col64.buf, n)
+    ws = am.Workspace(n)
+    stats = am.device_words(2)
+    tot = am.device_words(1)
+    out64 = am.Column(n, 64, am.BASELINE)
+    outs = am.Column(n, 11, am.SLAB)
+    sel.run()
+    sel8.run()
+    red.run()
+    am.compressstore(col64, 64, am.BASELINE, mask=mask, ws=ws, out=out64, stats=stats)
+    am.sum(col64, ws=ws, out=tot)
+    torch.cuda.synchronize()
+    k = int(am.u64(sel.count)[0])
+    same = (k == int(am.u64(stats)[0]) == int(am.u64(sel8.count)[0])
+            and int(am.u64(red.out)[0]) == int(am.u64(tot)[0])
+            and bool(torch.equal(sel.out[:k], out64.buf[:k])) and bool(torch.equal(sel8.out[:k], out64.buf[:k])))
+    mw = (n + 63) // 64 * 8
+    rows = []
+
+    def rec(op, fn, byt):
+        ms = timer.median_ms(fn, reps)
+        rows.append({"op": op, "n": n, "selected": k, "ms": round(ms, 4), "GBs": round(gbytes(byt, ms), 1),
+                     "pct_peak": round(100 * gbytes(byt, ms) / PEAK_GBS, 1)})
+
+    rec("hipcub DeviceSelect::Flagged, u64 + bitmask -> u64", sel.run, n * 8 + mw + k * 8)
+    rec("hipcub DeviceSelect::Flagged, u64 + flag bytes -> u64", sel8.run, n * 8 + n + k * 8)
+    rec("compressstore(mask), u64 -> u64", lambda: am.compressstore(col64, 64, am.BASELINE, mask=mask, ws=ws,
+                                                                    out=out64, stats=stats), n * 8 + mw + k * 8)
+    rec("compressstore(mask), 11-bit slab -> slab", lambda: am.compressstore(slab, 11, am.SLAB, mask=mask, ws=ws,
+                                                                             out=outs, stats=stats),
+        slab.words * 8 + mw + am.packed_words(am.SLAB, 11, k) * 8)
+    rec("hipcub DeviceReduce::Sum, u64", red.run, n * 8)
+    rec("sum, u64", lambda: am.sum(col64, ws=ws, out=tot), n * 8)
+    rec("sum, 11-bit slab", lambda: am.sum(slab, ws=ws, out=tot), slab.words * 8)
+    # time against hipCUB's faster form of the same operation
+    bsel = min(rows[0]["ms"], rows[1]["ms"])
+    for r in rows:
+        base = bsel if "Select" in r["op"] or "compressstore" in r["op"] else rows[4]["ms"]
+        r["speedup_vs_hipcub"] = round(base / r["ms"], 2)
+    del col64, slab, out64, outs, sel, sel8, flags8, red
+    torch.cuda.empty_cache()
+    return {"results_match": same, "rows": rows}
+
+
 def batched_leg(am, torch, reps):
     """512 columns, lengths log-uniform in [4K, 64M], widths U[3,31], values
     uniform / Zipf(1.2) / sorted+noise; per column filter -> compressstore ->
@@ -598,6 +658,9 @@ def main():
     if "batched" in legs:
         print("batched", file=sys.stderr, flush=True)
         extra["batched"] = batched_leg(am, torch, args.reps)
+    if "libbase" in legs:
+        print("library baseline (hipCUB)", file=sys.stderr, flush=True)
+        extra["library_baseline"] = libbase_leg(am, torch, timer, args.reps)
     # measured copy rates on this box, 1 GiB device to device: the library's
     # streaming copy (16-byte non-temporal loads and stores) and torch's copy_
     src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")

@@ -1110,3 +1110,32 @@ def test_matmul_plan_and_graph_replay(am, oracle_mod):
     torch.cuda.synchronize()
     for (cp, ci, ob), C, (_, _, M, N, K) in zip(outs, exps, probs):
         assert np.array_equal(ci.cpu().numpy().reshape(M, N).astype(np.int64), C)
+
+
+def test_hipcub_baseline_matches_library(am, oracle_mod):
+    """The GPU library baselines bench.py times (hipCUB select/reduce over one
+    u64 per element; the reference's cub baselines,
+    src/benchs/bench_compressstore_baseline_cub.cu:29-35 and
+    src/benchs/bench_hop_baseline_cub.cu:32-38) compute what the library's
+    compaction and sum compute on the same data, and what numpy computes."""
+    import torch
+    from amethyst_amd import baseline as hb
+    for n, s in ((1, 1.0), (4097, 0.5), (100_003, 0.3)):
+        say("hipcub baseline", n, s)
+        col = am.gen_uniform(7, n, 11, am.BASELINE)
+        mask = am.gen_mask_bernoulli(8, n, s)
+        sel = hb.SelectFlagged(col.buf, mask, n)
+        sel.run()
+        red = hb.ReduceSum(col.buf, n)
+        red.run()
+        out, stats, _ = am.compressstore(col, 64, am.BASELINE, mask=mask)
+        tot = am.sum(col)
+        torch.cuda.synchronize()
+        vals = col.host_words()
+        bits = np.unpackbits(am.u64(mask)[:(n + 63) // 64].view(np.uint8), bitorder="little")[:n].astype(bool)
+        exp = vals[bits]
+        k = int(am.u64(stats)[0])
+        assert k == len(exp) == int(am.u64(sel.count)[0]), (n, k, len(exp))
+        assert np.array_equal(am.u64(sel.out)[:k], exp), n
+        assert np.array_equal(out.host_words()[:k], exp), n
+        assert int(am.u64(red.out)[0]) == int(am.u64(tot)[0]) == int(vals.sum(dtype=np.uint64)), n
