@@ -25,6 +25,8 @@
 // Results are exact u64 sums and counts, in ascending key order, the same
 // outputs as the small-key path; the integer sums do not depend on the
 // order in which pairs land in a partition.
+#include <type_traits>
+
 #include "am_device.hpp"
 #include "am_host.hpp"
 
@@ -335,6 +337,118 @@ __global__ __launch_bounds__(GW_TH) void k_gbw_scatter_sorted(WideArgs a) { gbw_
 template <int KK, int VK>
 __global__ __launch_bounds__(GW_TH) void k_gbw_scatter_direct(WideArgs a) { gbw_scatter<KK, VK, false>(a); }
 
+// More than GW_SORTP partitions (keys of 21..24 bits, 512..4096
+// partitions): a block of GW_BTH threads counting-sorts tiles of GW_BIG
+// rows in LDS, so a partition's pairs of the tile leave as one run of about
+// GW_BIG / P pairs -- one store per row was one memory transaction per row. The block covers the same rows as row
+// block b of the histogram (blockIdx.x = b), so its starting places are
+// hist[p][b].
+constexpr uint32_t GW_BTH = 1024;
+constexpr uint32_t GW_BIG = 8192; // 8 rows per thread (16 spill at 1024 threads)
+
+__device__ inline uint32_t block_excl_u32_1024(uint32_t v, uint32_t* s_w, uint32_t* total)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t x = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d);
+        if (lane >= d) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t w = 0; w < GW_BTH / 64; w++) {
+        pre += w < wave ? s_w[w] : 0;
+        tot += s_w[w];
+    }
+    *total = tot;
+    return pre + x - v;
+}
+
+template <int KK, int VK, bool P64>
+__global__ __launch_bounds__(GW_BTH) void k_gbw_scatter_big(WideArgs a)
+{
+    typedef typename std::conditional<P64, uint64_t, uint32_t>::type PairT;
+    constexpr uint32_t TB = GW_BIG, RB = TB / GW_BTH;
+    constexpr uint32_t PPT = GW_MAXP / GW_BTH; // partition counters per thread in the scan
+    __shared__ PairT s_pair[TB];
+    __shared__ uint16_t s_part[TB];
+    __shared__ uint32_t cnt[GW_MAXP], loff[GW_MAXP], gpos[GW_MAXP];
+    __shared__ uint32_t s_w[GW_BTH / 64];
+    const uint32_t tid = threadIdx.x, P = a.P, LB = a.LB, vb = a.vbits;
+    for (uint32_t p = tid; p < P; p += GW_BTH) gpos[p] = (uint32_t)a.pbase[p] + a.hist[(uint64_t)p * a.G + blockIdx.x];
+    const uint64_t lm = lowmask(LB);
+    const uint64_t n = a.keys.n;
+    const uint64_t r0 = (uint64_t)blockIdx.x * a.rpb;
+    const uint64_t r1 = r0 + a.rpb < n ? r0 + a.rpb : n;
+    for (uint64_t t0 = r0; t0 < r1; t0 += TB) {
+        const TileReader<KK> rk(a.keys, t0);
+        const TileReader<VK> rv(a.vals, t0);
+        const uint64_t rem = r1 - t0;
+        // rows in groups of 8 (loads of a group in flight together), kept as
+        // (partition, pair): 2 registers a row instead of 4
+        uint32_t part[RB];
+        PairT pv[RB];
+#pragma unroll
+        for (uint32_t h = 0; h < RB; h += 8) {
+            uint64_t kv[8], vv[8];
+#pragma unroll
+            for (uint32_t r = 0; r < 8; r++) {
+                const uint32_t i = (h + r) * GW_BTH + tid;
+                kv[r] = i < rem ? rk.get(i) : 0;
+                vv[r] = i < rem ? rv.get(i) : 0;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < 8; r++) {
+                const uint32_t i = (h + r) * GW_BTH + tid;
+                const bool ok = i < rem && (!a.use_pred || a.pr(vv[r]));
+                part[h + r] = ok ? (uint32_t)(kv[r] >> LB) : 0xFFFFFFFFu;
+                pv[h + r] = (PairT)(((kv[r] & lm) << vb) | vv[r]);
+            }
+            asm volatile("" ::: "memory"); // (the next group's loads after this group's registers are freed)
+        }
+        for (uint32_t p = tid; p < P; p += GW_BTH) cnt[p] = 0;
+        __syncthreads();
+        uint32_t rank[RB];
+#pragma unroll
+        for (uint32_t r = 0; r < RB; r++) rank[r] = part[r] != 0xFFFFFFFFu ? atomicAdd(&cnt[part[r]], 1u) : 0;
+        __syncthreads();
+        // exclusive offsets of the tile's partitions, PPT consecutive per thread
+        uint32_t c[PPT], cs = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < PPT; q++) {
+            const uint32_t p = tid * PPT + q;
+            c[q] = p < P ? cnt[p] : 0;
+            cs += c[q];
+        }
+        uint32_t tot;
+        uint32_t off = block_excl_u32_1024(cs, s_w, &tot);
+#pragma unroll
+        for (uint32_t q = 0; q < PPT; q++) {
+            const uint32_t p = tid * PPT + q;
+            if (p < P) loff[p] = off;
+            off += c[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < RB; r++) {
+            if (part[r] == 0xFFFFFFFFu) continue;
+            const uint32_t d = loff[part[r]] + rank[r];
+            s_pair[d] = pv[r];
+            s_part[d] = (uint16_t)part[r];
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < tot; i += GW_BTH) {
+            const uint32_t p = s_part[i];
+            ((PairT*)a.pairs)[(uint64_t)gpos[p] + (i - loff[p])] = s_pair[i];
+        }
+        __syncthreads();
+        for (uint32_t p = tid; p < P; p += GW_BTH) gpos[p] += cnt[p];
+        __syncthreads();
+    }
+}
+
 // one block per chunk of GW_C pairs of one partition: one packed LDS word
 // per low key, (count << SB) | sum with SB = vbits + 15 -- a chunk's sum
 // fits SB bits and its count (<= 2^15) the 64 - SB >= 17 above, so a row
@@ -636,15 +750,32 @@ int am_groupby_wide(const Col& ck, const Col& cv, const am_predicate* value_pred
         case K_SLAB: hipLaunchKernelGGL((KERNEL<KK, K_SLAB>), dim3(a.G), dim3(GW_TH), 0, s, a); break;       \
         default: hipLaunchKernelGGL((KERNEL<KK, K_BITSTREAM>), dim3(a.G), dim3(GW_TH), 0, s, a); break;      \
     }
+#define GW_KVB(P64)                                                                                          \
+    switch (ck.kind) {                                                                                       \
+        case K_STRIDED: GW_VB(K_STRIDED, P64); break;                                                        \
+        case K_SLAB: GW_VB(K_SLAB, P64); break;                                                              \
+        default: GW_VB(K_BITSTREAM, P64); break;                                                             \
+    }
+#define GW_VB(KK, P64)                                                                                       \
+    switch (cv.kind) {                                                                                       \
+        case K_STRIDED: hipLaunchKernelGGL((k_gbw_scatter_big<KK, K_STRIDED, P64>), dim3(a.G), dim3(GW_BTH), 0, s, a); break; \
+        case K_SLAB: hipLaunchKernelGGL((k_gbw_scatter_big<KK, K_SLAB, P64>), dim3(a.G), dim3(GW_BTH), 0, s, a); break;       \
+        default: hipLaunchKernelGGL((k_gbw_scatter_big<KK, K_BITSTREAM, P64>), dim3(a.G), dim3(GW_BTH), 0, s, a); break;      \
+    }
     GW_KV(k_gbw_hist)
     hipLaunchKernelGGL(k_gbw_scan_part, dim3(a.P), dim3(GW_TH), 0, s, a);
     hipLaunchKernelGGL(k_gbw_scan_top, dim3(1), dim3(GW_TH), 0, s, a);
     if (a.P <= GW_SORTP) {
         GW_KV(k_gbw_scatter_sorted)
     }
-    else {
-        GW_KV(k_gbw_scatter_direct)
+    else if (a.pair64) {
+        GW_KVB(true)
     }
+    else {
+        GW_KVB(false)
+    }
+#undef GW_VB
+#undef GW_KVB
 #undef GW_V
 #undef GW_KV
     hipLaunchKernelGGL(k_gbw_agg, dim3((uint32_t)chunks_max), dim3(GW_TH), 0, s, a);
