@@ -62,6 +62,34 @@ struct WideArgs {
 
 __device__ inline uint32_t gw_width(uint64_t v) { return v ? 64 - (uint32_t)__clzll(v) : 1; }
 
+// LDS counter increments where many lanes may hit one counter (a Zipf key's
+// partition): the active lanes that share the first active lane's counter
+// take one atomic for all of them, the others one each. lds_add_agg adds 1
+// per lane; lds_inc_agg returns each lane's slot (the counter before its own
+// increment, lanes of the shared counter in lane order).
+__device__ inline void lds_add_agg(uint32_t* c, uint32_t p)
+{
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
+    const uint64_t same = ballot(p == p0);
+    if (p == p0) {
+        if (mbcnt(same) == 0) atomicAdd(&c[p0], (uint32_t)__popcll(same));
+    }
+    else atomicAdd(&c[p], 1u);
+}
+__device__ inline uint32_t lds_inc_agg(uint32_t* c, uint32_t p)
+{
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)p);
+    const uint64_t same = ballot(p == p0);
+    uint32_t r;
+    if (p == p0) {
+        uint32_t b = 0;
+        if (mbcnt(same) == 0) b = atomicAdd(&c[p0], (uint32_t)__popcll(same));
+        r = (uint32_t)__builtin_amdgcn_readfirstlane((int)b) + mbcnt(same); // (the first active lane leads)
+    }
+    else r = atomicAdd(&c[p], 1u);
+    return r;
+}
+
 __device__ inline uint64_t gw_shfl_up(uint64_t v, uint32_t d)
 {
     const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, d), hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), d);
@@ -190,7 +218,7 @@ __global__ __launch_bounds__(GW_TH) void k_gbw_hist(WideArgs a)
     for (uint32_t p = threadIdx.x; p < a.P; p += GW_TH) h[p] = 0;
     __syncthreads();
     const uint32_t LB = a.LB;
-    gw_rows<KK, VK, false>(a, [&](uint64_t k, uint64_t) { atomicAdd(&h[(uint32_t)(k >> LB)], 1u); });
+    gw_rows<KK, VK, false>(a, [&](uint64_t k, uint64_t) { lds_add_agg(h, (uint32_t)(k >> LB)); });
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < a.P; p += GW_TH) a.hist[(uint64_t)p * a.G + blockIdx.x] = h[p];
 }
@@ -330,7 +358,8 @@ __device__ inline void gbw_scatter(const WideArgs& a)
         for (uint32_t r = 0; r < GW_ROWS; r++) {
             const uint32_t i = r * GW_TH + threadIdx.x;
             const bool ok = i < rem && (!a.use_pred || a.pr(vv[r]));
-            rk_[r] = ok ? atomicAdd(&lh[(uint32_t)(kv[r] >> LB)], 1u) : 0xFFFFFFFFu;
+            rk_[r] = 0xFFFFFFFFu;
+            if (ok) rk_[r] = lds_inc_agg(lh, (uint32_t)(kv[r] >> LB));
         }
         __syncthreads();
         // exclusive offsets of the tile's partitions (one per thread, GW_SORTP == GW_TH)
