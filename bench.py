@@ -50,7 +50,7 @@ def parse():
     return p.parse_args()
 
 
-CASES = ("headline", "compress_slab_w4", "compress_slab_w11", "pipeline", "copy_1gib")
+CASES = ("headline", "compress_slab_w4", "compress_slab_w11", "pipeline", "copy_1gib", "groupby_u24", "groupby_z16")
 
 
 def gbytes(b, ms):
@@ -536,8 +536,9 @@ def batched_leg(am, torch, reps):
 def run_case(am, torch, name, steps):
     """One workload, `steps` launches, for profiling: the headline compaction,
     fused filter + compressstore into same-width slabs at 50 % selectivity
-    (w = 4, 11), the 512-column batched chain (one launch), or a plain 1 GiB
-    device copy (the reference point of the HBM byte counters)."""
+    (w = 4, 11), the 512-column batched chain (one launch), the wide-key
+    group-by (24-bit uniform / 16-bit Zipf keys), or a plain 1 GiB device
+    copy (the reference point of the HBM byte counters)."""
     n = N_FULL
     if name == "copy_1gib":
         src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
@@ -562,6 +563,26 @@ def run_case(am, torch, name, steps):
         byt = sum(a.words * 8 + b.words * 8 + am.packed_words(o.fmt, o.bits, int(x[0])) * 8
                   for (a, b, _), o, x in zip(cols, bt.outs, st))
         fn = bt.run
+    elif name in ("groupby_u24", "groupby_z16"):
+        # wide-key group-by (partitioned path): 2^28 rows, 20-bit uniform
+        # values, 24-bit uniform keys or 16-bit Zipf(1.1) ranks
+        vals = am.gen_uniform(43, n, 20, am.SLAB)
+        if name == "groupby_u24":
+            kb, keys = 24, am.gen_uniform(44, n, 24, am.SLAB)
+        else:
+            p16 = 1.0 / np.arange(1, 65537) ** 1.1
+            c16 = np.cumsum(p16 / p16.sum())
+            thr16 = np.minimum(np.floor(c16 * 4294967296.0), 4294967295).astype(np.uint64)
+            thr16[-1] = 4294967295
+            kb, keys = 16, am.gen_from_cdf(45, thr16.astype(np.uint32), n, 16, am.SLAB)
+        wsw = am.Workspace.for_groupby(n, kb, 20)
+        fn = lambda: am.groupby(keys, vals, ws=wsw)
+        _, _, _, res = fn()
+        torch.cuda.synchronize()
+        r = am.u64(res)
+        g = int(r[0])
+        byt = keys.words * 8 + vals.words * 8 + (am.packed_words(am.SLAB, kb, g) + am.packed_words(
+            am.SLAB, int(r[1]), g) + am.packed_words(am.SLAB, int(r[2]), g)) * 8
     else:
         w = {"headline": 11, "compress_slab_w4": 4, "compress_slab_w11": 11}[name]
         col = am.gen_uniform(42, n, w, am.SLAB)
