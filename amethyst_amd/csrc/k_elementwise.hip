@@ -7,13 +7,16 @@
 // compile-time function pointer masked to the out-width (src/config.cuh:5-42).
 //
 // Here one lane also owns one output word (so every store is a coalesced,
-// whole-word write and dead/padding bits are written as zero). Slab and
-// container columns go word-per-lane (k_binary_words: fields at wave-uniform
-// shifts); bitstream columns decode each element straight from global memory
-// through a per-block TileReader: neighbouring lanes touch neighbouring input words, so each load
-// instruction covers one or two 128-byte lines and the L1 absorbs the reuse
-// between lanes. Any input format x any output format; the operator is a
-// template argument so the switch folds away.
+// whole-word write and dead/padding bits are written as zero). Columns whose
+// input and output words hold the same elements go word-per-lane
+// (k_binary_aligned); unaligned slab pairs and bitstream inputs are staged in
+// LDS per block, and a lane reads one 64-bit window of fields per input
+// column and output word, every field then a shift by a wave-uniform amount
+// (k_binary_stream); pairs with a container side read two words per lane
+// from global memory (k_binary_words). Conversion and the remaining cases
+// decode each element through a per-block TileReader (k_elementwise). Any
+// input format x any output format; the operator is a template argument so
+// the switch folds away.
 #include "am_device.hpp"
 #include "am_host.hpp"
 
@@ -102,7 +105,8 @@ __global__ __launch_bounds__(256) void k_elementwise(Col a, Col b, Col out)
 // operator field by field at wave-uniform shifts. Otherwise the lane's first
 // input word and field come from a per-block base and a 32-bit reciprocal
 // multiply, and it reads two words of each input (eps_o <= eps_i: an output
-// word's elements span at most two input words).
+// word's elements span at most two input words), joined into one window of
+// fields.
 struct BinWords {
     Col a, b, out;
     uint32_t fi, eps_i, fo, eps_o;
@@ -190,16 +194,18 @@ __global__ __launch_bounds__(256) void k_binary_words(BinWords g)
         if (q >= g.words_o) break;
         const uint64_t e0 = q * eo;
         const uint32_t cnt = (uint32_t)(n - e0 < eo ? n - e0 : eo);
+        // the word's eo fields side by side (eo * fi <= 64 bits): the fields
+        // of x0 from s0 on, then those of x1 (dead bits of x0 cleared)
+        const uint32_t s = s0[k], lo = (ei - s) * fi; // s < ei; lo < 64 when x1 is needed
+        const bool two = s + eo > ei;
+        const uint64_t keep = two ? lowmask(lo) : ~0ull;
+        const uint64_t X = ((x0[k] >> (s * fi)) & keep) | (two ? x1[k] << lo : 0);
+        const uint64_t Y = ((y0[k] >> (s * fi)) & keep) | (two ? y1[k] << lo : 0);
         uint64_t word = 0;
-        for (uint32_t j = 0, so = 0; j < eo; j++, so += fo) {
-            const uint32_t s = s0[k] + j;
-            const bool hi = s >= ei;
-            const uint32_t sh = (hi ? s - ei : s) * fi;
-            const uint64_t va = ((hi ? x1[k] : x0[k]) >> sh) & im, vb = ((hi ? y1[k] : y0[k]) >> sh) & im;
-            const uint64_t r = binop<OP>(va, vb) & om;
-            word |= (j < cnt ? r : 0) << so;
-        }
-        st_nt(g.out.p + q, word);
+        for (uint32_t j = 0, si = 0, so = 0; j < eo; j++, si += fi, so += fo)
+            word |= (binop<OP>((X >> si) & im, (Y >> si) & im) & om) << so;
+        // fields past the column's end (its last word) are zero
+        st_nt(g.out.p + q, cnt == eo ? word : word & lowmask(cnt * fo));
     }
 }
 
@@ -217,11 +223,9 @@ static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s
     g.fo = field(o);
     g.eps_o = 64 / g.fo;
     if (g.eps_o > g.eps_i) return false; // narrower output: the general kernel
-    // unaligned, input with the stream layout (width divides 64): the stream
-    // kernel measured faster (w=4 -> 5: 0.165 vs 0.219 ms; w=32 -> 33: 0.73 vs 0.90)
-    if (g.eps_o != g.eps_i && a.kind == K_SLAB && 64 % a.bits == 0 && a.bits <= 32 &&
-        (o.kind == K_SLAB || o.kind == K_BITSTREAM))
-        return false;
+    // slab -> slab with fewer fields per output word: the staged kernel
+    // (measured faster: w=4 -> 5 0.101 vs 0.144 ms, w=10 -> 11 0.21 vs 0.26)
+    if (g.eps_o != g.eps_i && a.kind == K_SLAB && o.kind == K_SLAB) return false;
     g.a.eps = g.eps_i;
     g.a.eps_magic = g.eps_i > 1 ? (uint32_t)((1ull << 32) / g.eps_i + 1) : 0;
     g.words_i = (a.n + g.eps_i - 1) / g.eps_i;
@@ -254,17 +258,25 @@ static bool launch_words(const Col& a, const Col& b, const Col& o, hipStream_t s
 // builds whole output words, taking each element's wi bits with one funnel
 // shift (v_alignbit) of two LDS dwords. Indices are block-relative 32-bit;
 // divisions by the out-width are a reciprocal multiply (exact below 2^24).
+// ISLAB: slab inputs with dead bits into a slab output of fewer fields per
+// word, staged the same way; a lane takes its output word's fields from two
+// staged words of each input (the window below) -- reading the words from
+// global memory per lane (k_binary_words) measured slower, its loads being
+// lane-strided.
 constexpr uint32_t BS_WORDS = 1024;                // output words per block (4 per lane)
 constexpr uint32_t BS_IN = BS_WORDS + 4;           // input words per column staged per block
 struct BinStream {
     Col a, b, out;
     uint32_t wi, wo, eps_o, magic_o; // magic_o: floor(2^32/wo)+1 (64-bit multiply)
+    uint32_t eps_i, magic_i;         // ISLAB: fields per input word, floor(2^32/eps_i)+1
+    uint32_t win_e, max_e;           // bitstream output: floor(64/wi), floor(63/wo)+2
     uint64_t words_i, words_o;
 };
 
-template <int OP, bool OSLAB>
+template <int OP, bool OSLAB, bool ISLAB>
 __global__ __launch_bounds__(256) void k_binary_stream(BinStream g)
 {
+    static_assert(OSLAB || !ISLAB, "slab inputs go to slab outputs");
     __shared__ uint64_t s_a[BS_IN + 1], s_b[BS_IN + 1];
     const uint32_t tid = threadIdx.x;
     const uint64_t n = g.out.n;
@@ -275,10 +287,10 @@ __global__ __launch_bounds__(256) void k_binary_stream(BinStream g)
     const uint64_t t0 = OSLAB ? q0 * g.eps_o : (q0 * 64) / wo;
     uint64_t tend = OSLAB ? qend * g.eps_o : (qend * 64 - 1) / wo + 1;
     if (tend > n) tend = n;
-    const uint64_t ib0 = (t0 * wi) >> 6;
-    uint64_t ib1 = (tend * wi + 63) >> 6;
+    const uint64_t ib0 = ISLAB ? t0 / g.eps_i : (t0 * wi) >> 6;
+    uint64_t ib1 = ISLAB ? (tend + g.eps_i - 1) / g.eps_i : (tend * wi + 63) >> 6;
     if (ib1 > g.words_i) ib1 = g.words_i;
-    const uint32_t nin = (uint32_t)(ib1 - ib0); // <= BS_IN (wo >= wi)
+    const uint32_t nin = (uint32_t)(ib1 - ib0); // <= BS_IN (wo >= wi; ISLAB: eps_o < eps_i)
     {
         // every load of the thread first (each LDS store would otherwise wait
         // for its own load: one memory round trip per step)
@@ -307,31 +319,98 @@ __global__ __launch_bounds__(256) void k_binary_stream(BinStream g)
     const uint32_t ne = (uint32_t)(tend - t0);                // elements the block may touch
     const uint32_t im = (uint32_t)g.a.mask;
     const uint64_t om = g.out.mask;
+    if (ISLAB) {
+        // word qr holds elements qr*eps_o .. +eps_o, starting in staged word dq
+        // at field s: the fields of that word from s on, then those of the
+        // next (eps_o * wi <= 64 bits, dead bits of the first word cleared)
+        const uint32_t eo = g.eps_o, ei = g.eps_i;
+        const uint32_t r0 = (uint32_t)(t0 - ib0 * ei); // field of element t0 in staged word 0
+        for (uint32_t k = 0; k < BS_WORDS / 256; k++) {
+            const uint32_t qr = k * 256 + tid;
+            if (q0 + qr >= qend) break;
+            const uint32_t e = qr * eo;
+            const uint32_t cnt = ne - e < eo ? ne - e : eo;
+            const uint32_t rel = r0 + e; // < 2^17
+            const uint32_t dq = __umulhi(rel, g.magic_i), s = rel - dq * ei; // dq + 1 <= nin
+            const uint32_t lo = (ei - s) * wi;   // < 64 when the next word is needed (s >= 1)
+            const bool two = s + eo > ei;
+            const uint64_t keep = two ? lowmask(lo) : ~0ull;
+            const uint64_t xa = ((s_a[dq] >> (s * wi)) & keep) | (two ? s_a[dq + 1] << lo : 0);
+            const uint64_t xb = ((s_b[dq] >> (s * wi)) & keep) | (two ? s_b[dq + 1] << lo : 0);
+            uint64_t word = 0;
+            for (uint32_t f = 0, si = 0, so = 0; f < eo; f++, si += wi, so += wo) { // si <= 64 - wi
+                const uint32_t va = (uint32_t)(xa >> si) & im, vb = (uint32_t)(xb >> si) & im;
+                word |= (binop<OP>((uint64_t)va, (uint64_t)vb) & om) << so;
+            }
+            st_nt(g.out.p + q0 + qr, cnt == eo ? word : word & lowmask(cnt * wo));
+        }
+        return;
+    }
+    if (OSLAB) {
+        // word qr holds elements qr*eps_o .. +eps_o, at most 64 input bits: one
+        // three-dword window per column gives all of them, each field then a
+        // shift by a wave-uniform amount (no LDS read per element)
+        const uint32_t eo = g.eps_o;
+        for (uint32_t k = 0; k < BS_WORDS / 256; k++) {
+            const uint32_t qr = k * 256 + tid;
+            if (q0 + qr >= qend) break;
+            const uint32_t e = qr * eo;
+            const uint32_t cnt = ne - e < eo ? ne - e : eo; // e < ne: the word exists
+            const uint32_t ib = rb + e * wi, j = ib >> 5, sh = ib & 31; // j + 2 <= 2 * nin + 1
+            const uint64_t xa = ((uint64_t)__builtin_amdgcn_alignbit(sa[j + 2], sa[j + 1], sh) << 32) |
+                                __builtin_amdgcn_alignbit(sa[j + 1], sa[j], sh);
+            const uint64_t xb = ((uint64_t)__builtin_amdgcn_alignbit(sb[j + 2], sb[j + 1], sh) << 32) |
+                                __builtin_amdgcn_alignbit(sb[j + 1], sb[j], sh);
+            uint64_t word = 0;
+            for (uint32_t f = 0, si = 0, so = 0; f < eo; f++, si += wi, so += wo) { // si <= 64 - wi
+                const uint32_t va = (uint32_t)(xa >> si) & im, vb = (uint32_t)(xb >> si) & im;
+                word |= (binop<OP>((uint64_t)va, (uint64_t)vb) & om) << so;
+            }
+            // fields past the column's end (its last word) are zero
+            st_nt(g.out.p + q0 + qr, cnt == eo ? word : word & lowmask(cnt * wo));
+        }
+        return;
+    }
+    // bitstream output: word qr's elements [e, e1) -- the first may start
+    // before the word -- number at most max_e; their input bits are read as
+    // two 64-bit windows, the first holding win_e = floor(64/wi) whole
+    // elements from e on, the second the (at most two) after them, so each
+    // field is a shift by a wave-uniform amount and only the output position
+    // differs per lane
     auto div_o = [&](uint32_t x) { return (uint32_t)(((uint64_t)x * g.magic_o) >> 32); };
+    auto window = [&](const uint32_t* sp, uint32_t ib) {
+        const uint32_t j = ib >> 5 < 2 * BS_IN - 1 ? ib >> 5 : 2 * BS_IN - 1, sh = ib & 31;
+        return ((uint64_t)__builtin_amdgcn_alignbit(sp[j + 2], sp[j + 1], sh) << 32) |
+               __builtin_amdgcn_alignbit(sp[j + 1], sp[j], sh);
+    };
+    const uint32_t m = g.win_e, me = g.max_e;
     for (uint32_t k = 0; k < BS_WORDS / 256; k++) {
         const uint32_t qr = k * 256 + tid;
         if (q0 + qr >= qend) break;
-        uint32_t e, e1;   // block-relative elements of the word, [e, e1)
-        int32_t pos;      // output bit of element e relative to the word
-        if (OSLAB) {
-            e = qr * g.eps_o;
-            e1 = e + g.eps_o;
-            pos = 0;
-        }
-        else {
-            const uint32_t ob = qr * 64 + base; // the word's first output bit past element t0's start
-            e = div_o(ob);
-            e1 = div_o(ob + 63) + 1;
-            pos = (int32_t)(e * wo) - (int32_t)ob;
-        }
+        const uint32_t ob = qr * 64 + base; // the word's first output bit past element t0's start
+        const uint32_t e = div_o(ob);
+        uint32_t e1 = div_o(ob + 63) + 1;
         if (e1 > ne) e1 = ne;
-        uint64_t word = 0;
-        for (; e < e1; e++, pos += (int32_t)wo) {
-            const uint32_t ib = rb + e * wi, j = ib >> 5, sh = ib & 31;
-            const uint32_t va = __builtin_amdgcn_alignbit(sa[j + 1], sa[j], sh) & im;
-            const uint32_t vb = __builtin_amdgcn_alignbit(sb[j + 1], sb[j], sh) & im;
-            const uint64_t r = binop<OP>((uint64_t)va, (uint64_t)vb) & om;
-            word |= pos >= 0 ? (r << pos) : (r >> (-pos));
+        const uint32_t cnt = e1 - e;                          // 1 .. max_e
+        const uint32_t p0 = ob - e * wo;                      // element e starts p0 bits before the word
+        const uint32_t ib = rb + e * wi;
+        const uint64_t xa = window(sa, ib), xb = window(sb, ib);
+        auto field = [&](uint64_t x, uint64_t y, uint32_t si) {
+            const uint32_t va = (uint32_t)(x >> si) & im, vb = (uint32_t)(y >> si) & im;
+            return binop<OP>((uint64_t)va, (uint64_t)vb) & om;
+        };
+        uint64_t word = field(xa, xb, 0) >> p0;
+        uint32_t f = 1, p = wo - p0; // output bit of element e + f
+        for (uint32_t si = wi; f < m && f < me; f++, si += wi, p += wo) {
+            const uint64_t r = field(xa, xb, si);
+            word |= (f < cnt ? r : 0) << p; // p < 64 for f < cnt
+        }
+        if (f < me) { // uniform: the elements past the first window
+            const uint64_t ya = window(sa, ib + m * wi), yb = window(sb, ib + m * wi);
+            for (uint32_t si = 0; f < me; f++, si += wi, p += wo) {
+                const uint64_t r = field(ya, yb, si);
+                word |= (f < cnt ? r : 0) << p;
+            }
         }
         st_nt(g.out.p + q0 + qr, word);
     }
@@ -342,7 +421,9 @@ static bool launch_stream(const Col& a, const Col& b, const Col& o, hipStream_t 
 {
     // (a slab whose width divides 64 has the stream layout too)
     const bool stream_in = a.kind == K_BITSTREAM || (a.kind == K_SLAB && 64 % a.bits == 0);
-    if (!stream_in || a.bits > 32 || o.bits < a.bits) return false;
+    // other slabs: into a slab of fewer fields per word
+    const bool slab_in = !stream_in && a.kind == K_SLAB && o.kind == K_SLAB && 64 / o.bits < 64 / a.bits;
+    if (!(stream_in || slab_in) || a.bits > 32 || o.bits < a.bits) return false;
     if (o.kind != K_BITSTREAM && o.kind != K_SLAB) return false;
     BinStream g{};
     g.a = a;
@@ -357,9 +438,14 @@ static bool launch_stream(const Col& a, const Col& b, const Col& o, hipStream_t 
     const uint64_t mo = (1ull << 32) / o.bits + 1;
     if (mo >> 32) return false; // out-width 1 (only possible with in-width 1)
     g.magic_o = (uint32_t)mo;
+    g.win_e = 64 / a.bits;
+    g.max_e = 63 / o.bits + 2;
+    g.eps_i = 64 / a.bits;
+    g.magic_i = (uint32_t)((1ull << 32) / g.eps_i + 1); // eps_i >= 2 when used
     const dim3 grid((uint32_t)((g.words_o + BS_WORDS - 1) / BS_WORDS));
-    if (o.kind == K_SLAB) hipLaunchKernelGGL((k_binary_stream<OP, true>), grid, dim3(256), 0, s, g);
-    else hipLaunchKernelGGL((k_binary_stream<OP, false>), grid, dim3(256), 0, s, g);
+    if (slab_in) hipLaunchKernelGGL((k_binary_stream<OP, true, true>), grid, dim3(256), 0, s, g);
+    else if (o.kind == K_SLAB) hipLaunchKernelGGL((k_binary_stream<OP, true, false>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_binary_stream<OP, false, false>), grid, dim3(256), 0, s, g);
     return true;
 }
 

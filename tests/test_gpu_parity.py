@@ -318,6 +318,27 @@ def test_binary_bitstream_multiblock(am, oracle_mod):
                 assert np.array_equal(out.host_words(), exp), (bits, op, ofmt)
 
 
+def test_binary_slab_multiblock(am, oracle_mod):
+    """Slab inputs over many blocks into slab outputs of fewer fields per word
+    (an output word's fields start mid-way through an input word and run into
+    the next), and into containers."""
+    rng = np.random.default_rng(27)
+    for bits in (3, 5, 6, 7, 10, 13, 21):
+        n = 300007
+        a = rand_vals(rng, n, bits)
+        b = rand_vals(rng, n, bits)
+        ca = upload(am, oracle_mod, a, bits, 1)
+        cb = upload(am, oracle_mod, b, bits, 1)
+        for op in (0, 1, 10):  # add, mul, sub
+            for ofmt in (1, 0):
+                ob = oracle_mod.outbits(op, bits)
+                say("binary-slab", bits, op, ofmt, ob)
+                out = am.binary(op, ca, cb, out_bits=ob, out_fmt=ofmt)
+                exp = oracle_mod.binary(op, 1, bits, oracle_mod.pack(1, bits, a), oracle_mod.pack(1, bits, b),
+                                        n, ofmt, ob)
+                assert np.array_equal(out.host_words(), exp), (bits, op, ofmt)
+
+
 def test_sum(am, oracle_mod):
     rng = np.random.default_rng(8)
     ws = am.Workspace(0)
