@@ -356,7 +356,9 @@ __global__ __launch_bounds__(256) void k_sum_swar(SumSwar g, uint64_t* partial)
 // summed by the same even/odd slot accumulation as k_sum_swar -- and at most
 // one more field starts below bit 64, taken alone. The last word (fields
 // past n) takes the per-field path. About 5 lane operations per element
-// instead of about 10: the kernel is bound by integer issue, not bytes.
+// instead of about 10. Block steps away from the end run branch-free, with
+// the next word's low half taken from the next lane (DPP) rather than
+// loaded a second time.
 __global__ __launch_bounds__(256) void k_sum_bits_swar(SumSwar g, uint64_t nbits, uint64_t* partial)
 {
     __shared__ uint64_t s_part[4];
@@ -367,8 +369,46 @@ __global__ __launch_bounds__(256) void k_sum_bits_swar(SumSwar g, uint64_t nbits
     uint64_t total = 0, ae = 0, ao = 0;
     uint32_t cnt = 0;
     const uint64_t gstride = (uint64_t)gridDim.x * 256 * SUM_UNROLL;
+    const uint32_t lane = threadIdx.x & 63;
     for (uint64_t qb = (uint64_t)blockIdx.x * 256 * SUM_UNROLL; qb < g.words; qb += gstride) {
         const uint32_t r0 = (uint32_t)((qb * 64) % w); // bits of an element before word qb's bit 0, mod w
+        if (qb + 256 * SUM_UNROLL < g.words) {
+            // every word of the step and the one after each exist: the next
+            // word's low half comes from the next lane (DPP wave shift), lane
+            // 63 loading it; no per-word branches
+            uint64_t wv[SUM_UNROLL];
+            uint32_t xv[SUM_UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+                const uint64_t q = qb + u * 256 + threadIdx.x;
+                wv[u] = __builtin_nontemporal_load(g.p + q);
+                xv[u] = lane == 63 ? (uint32_t)g.p[q + 1] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < SUM_UNROLL; u++) {
+                const uint32_t d = u * 256 + threadIdx.x;
+                const uint32_t xo = r0 + d * 64; // < 2^17
+                const uint32_t rx = xo - (uint32_t)(((uint64_t)xo * wmagic) >> 32) * w;
+                const uint32_t off = rx ? w - rx : 0; // first element start in the word (< 32)
+                const uint32_t l0 = (uint32_t)wv[u], h0 = (uint32_t)(wv[u] >> 32);
+                const uint32_t nx = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)l0, 0x130, 0xf, 0xf, false); // lane+1
+                const uint32_t l1 = lane == 63 ? xv[u] : nx;
+                const uint64_t x = ((uint64_t)__builtin_amdgcn_alignbit(l1, h0, off) << 32) |
+                                   __builtin_amdgcn_alignbit(h0, l0, off);
+                ae += x & g.E;
+                ao += (x >> w) & g.O;
+                if (g.has_top) total += (x >> g.top_sh) & vmask;
+                const uint32_t pe = off + F * w; // >= 33 (F*w > 64 - w)
+                total += pe < 64 ? (__builtin_amdgcn_alignbit(l1, h0, pe - 32) & vm) : 0;
+            }
+            cnt += SUM_UNROLL;
+            if (cnt >= g.K) {
+                total += swar_slots(ae, w, nslots) + swar_slots(ao, w, nslots);
+                ae = ao = 0;
+                cnt = 0;
+            }
+            continue;
+        }
         uint64_t wv[SUM_UNROLL], nv[SUM_UNROLL];
 #pragma unroll
         for (uint32_t u = 0; u < SUM_UNROLL; u++) {
