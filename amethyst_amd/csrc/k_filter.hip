@@ -566,8 +566,20 @@ __global__ __launch_bounds__(256) void k_sum_u64(const uint64_t* p, uint64_t n, 
 __global__ __launch_bounds__(256) void k_sum_final(const uint64_t* partial, uint32_t nparts, uint64_t* out)
 {
     __shared__ uint64_t s_part[4];
+    // eight loads per thread in flight per step (a load-add loop waited one
+    // round trip per partial: 5.6 us for 4096 of them)
+    constexpr uint32_t U = 8;
     uint64_t acc = 0;
-    for (uint32_t i = threadIdx.x; i < nparts; i += 256) acc += partial[i];
+    for (uint32_t i0 = 0; i0 < nparts; i0 += 256 * U) {
+        uint64_t v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * 256 + threadIdx.x;
+            v[u] = i < nparts ? partial[i] : 0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) acc += v[u];
+    }
     acc = wave_sum_u64(acc);
     if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
     __syncthreads();

@@ -50,7 +50,8 @@ def parse():
     return p.parse_args()
 
 
-CASES = ("headline", "compress_slab_w4", "compress_slab_w11", "pipeline", "copy_1gib", "groupby_u24", "groupby_z16")
+CASES = ("headline", "compress_slab_w4", "compress_slab_w11", "pipeline", "copy_1gib", "groupby_u24", "groupby_z16",
+         "add_slab_w4", "add_bitstream_w11", "sum_bitstream_w11")
 
 
 def gbytes(b, ms):
@@ -537,10 +538,23 @@ def run_case(am, torch, name, steps):
     """One workload, `steps` launches, for profiling: the headline compaction,
     fused filter + compressstore into same-width slabs at 50 % selectivity
     (w = 4, 11), the 512-column batched chain (one launch), the wide-key
-    group-by (24-bit uniform / 16-bit Zipf keys), or a plain 1 GiB device
-    copy (the reference point of the HBM byte counters)."""
+    group-by (24-bit uniform / 16-bit Zipf keys), ADD of two w=4 slabs into a
+    w=5 slab or of two 11-bit bitstreams into a 12-bit bitstream, the sum of
+    an 11-bit bitstream, or a plain 1 GiB device copy (the reference point of
+    the HBM byte counters)."""
     n = N_FULL
-    if name == "copy_1gib":
+    if name in ("add_slab_w4", "add_bitstream_w11"):
+        w, fmt = (4, am.SLAB) if name == "add_slab_w4" else (11, am.BITSTREAM)
+        a, b = am.gen_uniform(11, n, w, fmt), am.gen_uniform(12, n, w, fmt)
+        out = am.Column(n, w + 1, fmt)
+        fn = lambda: am.binary(am.ADD, a, b, out=out)
+        byt = (a.words + b.words + out.words) * 8
+    elif name == "sum_bitstream_w11":
+        a = am.gen_uniform(21, n, 11, am.BITSTREAM)
+        ws, s = am.Workspace(0), am.device_words(1)
+        fn = lambda: am.sum(a, ws=ws, out=s)
+        byt = a.words * 8 + 8
+    elif name == "copy_1gib":
         src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
         fn, byt = (lambda: dst.copy_(src)), 2 << 30
