@@ -233,6 +233,39 @@ __device__ inline uint32_t select_fields(uint64_t x, uint32_t eps, uint32_t fw, 
     return (__builtin_bitreverse32(gr) >> (32 - eps)) ^ tp.inv;
 }
 
+// Selection bits of a word of eps = 64/fw fields, fw = 2, 4 or 8 (no dead
+// bits), for a range from 0 (v < sp1): even and odd fields in 2fw-bit slots
+// (E: their value bits), v + (2^fw - sp1) (K, per slot) carries into the
+// slot's bit fw (G) iff v >= sp1, two 64-bit adds for all fields instead of
+// a compare per field; the not-selected bits then sit in 2-bit groups at a
+// stride of 2fw (even field low, odd field high), gathered by log steps.
+__device__ inline uint32_t select_fields_pow2(uint64_t x, uint32_t fw, uint64_t E, uint64_t K, uint64_t G, uint32_t inv)
+{
+    const uint64_t ne = ((x & E) + K) & G;
+    const uint64_t no = (((x >> fw) & E) + K) & G;
+    uint64_t c = (ne >> fw) | (no >> (fw - 1));
+    uint32_t all;
+    if (fw == 2) {
+        c = (c | (c >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+        c = (c | (c >> 4)) & 0x00FF00FF00FF00FFull;
+        c = (c | (c >> 8)) & 0x0000FFFF0000FFFFull;
+        c = (c | (c >> 16)) & 0xFFFFFFFFull;
+        all = 0xFFFFFFFFu;
+    }
+    else if (fw == 4) {
+        c = (c | (c >> 6)) & 0x000F000F000F000Full;
+        c = (c | (c >> 12)) & 0x000000FF000000FFull;
+        c = (c | (c >> 24)) & 0xFFFFull;
+        all = 0xFFFFu;
+    }
+    else {
+        c = (c | (c >> 14)) & 0x0000000F0000000Full;
+        c = (c | (c >> 28)) & 0xFFull;
+        all = 0xFFu;
+    }
+    return ((uint32_t)c ^ all) ^ inv;
+}
+
 // ------------------------------------------------------------ wave64 helpers
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }

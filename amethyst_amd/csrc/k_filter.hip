@@ -158,7 +158,10 @@ __global__ __launch_bounds__(256) void k_filter_stream(FilterStreamArgs a)
 
 // Word-per-lane filter for slab and container inputs: a block owns a tile
 // of 256*R input words; each lane issues its R word loads at once and forms
-// each word's selection bits with select_fields (4 VALU per field). Mask
+// each word's selection bits with select_fields (4 VALU per field), or, for
+// fields of 2, 4 or 8 bits under a range from 0 or up to the top value (<,
+// <=, >, >=, and == / != of the top value), with select_fields_pow2 (two
+// 64-bit adds for all fields, then a log-step gather of the bits). Mask
 // words: one field per word -> the chunk's ballot; a power-of-two number of
 // fields per word -> the lanes feeding one mask word are an aligned segment
 // of 64/eps lanes, combined with a DPP OR-scan and stored by its last lane;
@@ -173,6 +176,8 @@ struct FilterWordsArgs {
     uint32_t fw, eps, R, pow2;
     uint64_t in_words;
     uint64_t* mask_out;
+    uint32_t swar, swinv;     // fields of 2, 4 or 8 bits, range from 0 or to the top: select_fields_pow2
+    uint64_t swE, swK, swG;   // its slot masks
 };
 
 constexpr uint32_t FW_RMAX = 8;
@@ -211,7 +216,9 @@ __global__ __launch_bounds__(256) void k_filter_words(FilterWordsArgs a)
     for (uint32_t r = 0; r < FW_RMAX; r++) {
         if (r >= R) continue;
         const uint32_t ci = wave * R + r;
-        uint32_t gg = WIDE ? (uint32_t)a.pr(x[r] & a.in.mask) : select_fields(x[r], eps, fw, bw, a.tp);
+        uint32_t gg = WIDE ? (uint32_t)a.pr(x[r] & a.in.mask)
+                           : (a.swar ? select_fields_pow2(x[r], fw, a.swE, a.swK, a.swG, a.swinv)
+                                     : select_fields(x[r], eps, fw, bw, a.tp));
         if (last) {
             const uint64_t e0 = (w0 + ci * 64 + lane) * eps;
             const uint32_t nv = e0 >= n ? 0 : (n - e0 < eps ? (uint32_t)(n - e0) : eps);
@@ -617,6 +624,20 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
         a.R = (uint32_t)(r > FW_RMAX ? FW_RMAX : r);
         if (const uint32_t tr = am_tuned(AM_KNOB_FILTER_WORDS, c.bits, in->format, 0)) a.R = tr > FW_RMAX ? FW_RMAX : tr;
         if (c.bits <= 32) a.tp = make_top_pred(pr, c.bits, a.eps);
+        const uint64_t vmax = lowmask(c.bits);
+        const bool from0 = a.tp.lo == 0, to_top = pr.lo && pr.span >= vmax - pr.lo; // [0, ..) or [lo, vmax]
+        if (c.bits <= 32 && c.bits == a.fw && (a.fw == 2 || a.fw == 4 || a.fw == 8) && (from0 || to_top)) {
+            // v < sp1 for every field at once (slots of 2fw bits); a range up
+            // to the top value is the complement of v < lo
+            const uint64_t sp1 = from0 ? ((uint64_t)a.tp.rg + 1) >> (32 - c.bits) : pr.lo; // 1 .. 2^fw
+            a.swinv = from0 ? a.tp.inv : a.tp.inv ^ (uint32_t)lowmask(a.eps);
+            for (uint32_t k = 0; k < a.eps / 2; k++) {
+                a.swE |= lowmask(a.fw) << (2 * a.fw * k);
+                a.swK |= ((1ull << a.fw) - sp1) << (2 * a.fw * k);
+                a.swG |= (1ull << a.fw) << (2 * a.fw * k);
+            }
+            a.swar = 1;
+        }
         a.mask_out = mask_out;
         const uint64_t TW = 256ull * a.R;
         const uint64_t TWB = (a.in_words + TW - 1) / TW;
