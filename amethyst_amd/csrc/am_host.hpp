@@ -109,6 +109,30 @@ inline TopPred make_top_pred(const Pred& p, uint32_t bits, uint32_t eps)
     return t;
 }
 
+// SWAR selection (select_fields_pow2) applies to fields of fw = 2, 4 or 8
+// bits with no dead bits (fw == bits) under a range from 0 or one ending at
+// the top value (the complement of v < lo): fills its slot masks and the
+// flip of the result; false when it does not apply.
+struct SwarSel {
+    uint64_t E, K, G;
+    uint32_t inv;
+};
+inline bool swar_select(const Pred& pr, const TopPred& tp, uint32_t bits, uint32_t fw, uint32_t eps, SwarSel* o)
+{
+    if (bits > 32 || bits != fw || (fw != 2 && fw != 4 && fw != 8)) return false;
+    const uint64_t vmax = lowmask(bits);
+    const bool from0 = tp.lo == 0, to_top = pr.lo && pr.span >= vmax - pr.lo; // [0, ..) or [lo, vmax]
+    if (!from0 && !to_top) return false;
+    const uint64_t sp1 = from0 ? ((uint64_t)tp.rg + 1) >> (32 - bits) : pr.lo; // 1 .. 2^fw: v < sp1
+    *o = SwarSel{0, 0, 0, from0 ? tp.inv : tp.inv ^ (uint32_t)lowmask(eps)};
+    for (uint32_t k = 0; k < eps / 2; k++) {
+        o->E |= lowmask(fw) << (2 * fw * k);
+        o->K |= ((1ull << fw) - sp1) << (2 * fw * k);
+        o->G |= (1ull << fw) << (2 * fw * k);
+    }
+    return true;
+}
+
 // The selected values of a predicate over `bits` <= 32-bit values as one
 // cyclic range [c0, c0 + m) (mod 2^bits), scaled to values top-aligned in 32
 // bits with arbitrary bits beneath: selected <=> (y - lo) mod 2^32 <= rg.

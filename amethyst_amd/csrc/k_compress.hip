@@ -67,6 +67,8 @@ struct CompressArgs {
     uint32_t lds16;   // input pointer 16-byte aligned: 16-byte global->LDS copies
     uint32_t out_vec; // output pointer aligned for 4-element vector stores
     TopPred tp;       // predicate over top-aligned fields (direct kernel)
+    uint32_t swar, swinv;   // 2/4/8-bit fields: select_fields_pow2 with these masks
+    uint64_t swE, swK, swG;
 };
 
 __device__ inline void store_container(const Col& o, uint64_t r, uint64_t v)
@@ -329,6 +331,9 @@ __global__ __launch_bounds__(CD_THREADS, (CdWaves<OK, OVF>::V)) void k_compress_
         }
         else if (WIDE)
             gg = a.pr(xr & a.in.mask);
+        else if (VB == 1 && a.swar) // (2/4/8-bit fields stage 1-byte values; other instances keep
+                                    // their code as it was: the branch had cost w=11 1-2 %)
+            gg = select_fields_pow2(xr, fw, a.swE, a.swK, a.swG, a.swinv);
         else
             gg = select_fields(xr, eps, fw, bw, a.tp);
         if (last) { // elements past n (only in the last tile)
@@ -808,7 +813,17 @@ static int compress_launch(const am_column* in, const uint64_t* mask, const am_p
             if (a.R < 1) a.R = 1;
             a.lds16 = ((uintptr_t)ci.p & 15) == 0;
             a.out_vec = ((uintptr_t)co.p & (4 * co.cbytes - 1)) == 0;
-            if (pred && !mask && ci.bits <= 32) a.tp = make_top_pred(a.pr, ci.bits, a.eps);
+            if (pred && !mask && ci.bits <= 32) {
+                a.tp = make_top_pred(a.pr, ci.bits, a.eps);
+                SwarSel sw;
+                if (swar_select(a.pr, a.tp, ci.bits, a.fw, a.eps, &sw)) {
+                    a.swar = 1;
+                    a.swinv = sw.inv;
+                    a.swE = sw.E;
+                    a.swK = sw.K;
+                    a.swG = sw.G;
+                }
+            }
         }
         a.in_words = (ci.n + a.eps - 1) / a.eps;
         const uint64_t TW = (uint64_t)256 * a.R; // 4 (data) waves x R chunks x 64 words

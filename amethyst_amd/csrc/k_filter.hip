@@ -624,19 +624,13 @@ extern "C" int am_filter(const am_column* in, am_predicate pred, uint64_t* mask_
         a.R = (uint32_t)(r > FW_RMAX ? FW_RMAX : r);
         if (const uint32_t tr = am_tuned(AM_KNOB_FILTER_WORDS, c.bits, in->format, 0)) a.R = tr > FW_RMAX ? FW_RMAX : tr;
         if (c.bits <= 32) a.tp = make_top_pred(pr, c.bits, a.eps);
-        const uint64_t vmax = lowmask(c.bits);
-        const bool from0 = a.tp.lo == 0, to_top = pr.lo && pr.span >= vmax - pr.lo; // [0, ..) or [lo, vmax]
-        if (c.bits <= 32 && c.bits == a.fw && (a.fw == 2 || a.fw == 4 || a.fw == 8) && (from0 || to_top)) {
-            // v < sp1 for every field at once (slots of 2fw bits); a range up
-            // to the top value is the complement of v < lo
-            const uint64_t sp1 = from0 ? ((uint64_t)a.tp.rg + 1) >> (32 - c.bits) : pr.lo; // 1 .. 2^fw
-            a.swinv = from0 ? a.tp.inv : a.tp.inv ^ (uint32_t)lowmask(a.eps);
-            for (uint32_t k = 0; k < a.eps / 2; k++) {
-                a.swE |= lowmask(a.fw) << (2 * a.fw * k);
-                a.swK |= ((1ull << a.fw) - sp1) << (2 * a.fw * k);
-                a.swG |= (1ull << a.fw) << (2 * a.fw * k);
-            }
+        SwarSel sw;
+        if (swar_select(pr, a.tp, c.bits, a.fw, a.eps, &sw)) {
             a.swar = 1;
+            a.swinv = sw.inv;
+            a.swE = sw.E;
+            a.swK = sw.K;
+            a.swG = sw.G;
         }
         a.mask_out = mask_out;
         const uint64_t TW = 256ull * a.R;
