@@ -110,6 +110,30 @@ def test_filter(am, oracle_mod):
                     assert np.array_equal(got, exp), (bits, n, fmt, op)
 
 
+def test_filter_pow2_fields_every_threshold(am, oracle_mod):
+    """2-, 4- and 8-bit slabs, whose fields are selected SWAR-wise under
+    ranges from 0 or up to the top value: every threshold of <, <=, >, >=,
+    and == / != of 0 and of the top value, in the filter and in the fused
+    compaction."""
+    rng = np.random.default_rng(29)
+    for bits in (2, 4, 8):
+        vmax = (1 << bits) - 1
+        n = 20011
+        vals = rand_vals(rng, n, bits)
+        col = upload(am, oracle_mod, vals, bits, 1)
+        cs = sorted({0, 1, vmax // 2, vmax - 1, vmax, vmax + 1} | (set(range(vmax + 2)) if bits <= 4 else set()))
+        cases = [(op, c) for op in (0, 1, 2, 3) for c in cs] + [(4, 0), (4, vmax), (5, 0), (5, vmax)]
+        for op, c in cases:
+            say("filter-pow2", bits, op, c)
+            got = am.u64(am.filter(col, op, c))[:(n + 63) // 64]
+            exp = oracle_mod.filter(1, bits, oracle_mod.pack(1, bits, vals), n, op, c)
+            assert np.array_equal(got, exp), (bits, op, c)
+        for op, c in ((0, vmax // 2 + 1), (3, vmax // 2), (2, vmax - 1)):
+            say("compress-pow2", bits, op, c)
+            _check_compress(am, oracle_mod, vals, bits, 1, bits, 1, pred=(op, c))
+            _check_compress(am, oracle_mod, vals, bits, 1, 32, 4, pred=(op, c))
+
+
 def _check_compress(am, oracle_mod, vals, bits, fi, ob, fo, mask_words=None, pred=None, cap=None, ws=None):
     n = len(vals)
     col = upload(am, oracle_mod, vals, bits, fi)
